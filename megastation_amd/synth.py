@@ -1,0 +1,222 @@
+"""Seeded synthetic uplink slots for the detection path.
+
+Constellations, bit order and the CSI convention follow the reference:
+
+* Gray QAM tables restated from ``src/operators/modulation/modulation.cu:22-72``
+  (same values, same index bit split); the table index is built from the bit
+  stream MSB-first (``modulation_kernel.cuh:54-58``: the first bit of a symbol
+  becomes the index MSB), so LLR ``j`` of the demod belongs to stream bit ``j``.
+* The reference 256-QAM table is the bitwise complement of what
+  ``demod256QAM`` decodes (``demodulation.h:54-67``); the generator therefore
+  indexes that table with the complemented index, which is the mapping the
+  demod (and 3GPP TS 38.211 with LLR > 0 <=> bit 0) expects.
+* CSI buffer ``A_g = H_g^H`` stored ``[G][B][U]`` (element (u,b,g) at
+  ``u + U*b + U*B*g``), as the pilot storer writes it
+  (``src/operators/fft/fft_callback.cuh:87-105``).
+* ``y = H x + n`` per subcarrier, stored ``[S][B][N]`` (subcarrier fastest,
+  ``src/baseband/buffers.cc:87-88``); unit-energy symbols, noise
+  ``CN(0, 10^(-SNR/10))``.
+
+Random numbers come from numpy's counter-based Philox generator, seeded.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def qam_table(Q: int) -> np.ndarray:
+    """complex64[2^Q] constellation indexed by the Q-bit symbol index."""
+    M = 1 << Q
+    t = np.zeros(M, dtype=np.complex64)
+    if Q == 2:
+        s = 1 / math.sqrt(2)
+        lv = [-s, s]
+        for i in range(4):
+            t[i] = complex(lv[i // 2], lv[i % 2])
+    elif Q == 4:
+        s = 1 / math.sqrt(10)
+        lv = [1 * s, 3 * s, -1 * s, -3 * s]
+        for i in range(16):
+            im_i = (((i >> 2) & 1) << 1) + (i & 1)
+            re_i = (((i >> 3) & 1) << 1) + ((i >> 1) & 1)
+            t[i] = complex(lv[re_i], lv[im_i])
+    elif Q == 6:
+        s = 1 / math.sqrt(42)
+        lv = [3 * s, 1 * s, 5 * s, 7 * s, -3 * s, -1 * s, -5 * s, -7 * s]
+        for i in range(64):
+            im_i = (((i >> 4) & 1) << 2) + (((i >> 2) & 1) << 1) + (i & 1)
+            re_i = (((i >> 5) & 1) << 2) + (((i >> 3) & 1) << 1) + ((i >> 1) & 1)
+            t[i] = complex(lv[re_i], lv[im_i])
+    elif Q == 8:
+        s = 1 / math.sqrt(170)
+        lv = [-15 * s, -13 * s, -9 * s, -11 * s, -1 * s, -3 * s, -7 * s, -5 * s,
+              15 * s, 13 * s, 9 * s, 11 * s, 1 * s, 3 * s, 7 * s, 5 * s]
+        for i in range(256):
+            j = (~i) & 0xFF  # complement: see module docstring
+            im_i = ((j & 0x40) >> 3) + ((j & 0x10) >> 2) + ((j & 0x4) >> 1) + (j & 1)
+            re_i = ((j & 0x80) >> 4) + ((j & 0x20) >> 3) + ((j & 0x8) >> 2) + ((j & 0x2) >> 1)
+            t[i] = complex(lv[re_i], lv[im_i])
+    else:
+        raise ValueError(f"unsupported modulation order {Q}")
+    return t
+
+
+def reference_table(Q: int) -> np.ndarray:
+    """The reference's own table for Q (for Q=8 *without* the complement),
+    used only to document defect D3 in the tests."""
+    if Q != 8:
+        return qam_table(Q)
+    s = 1 / math.sqrt(170)
+    lv = [-15 * s, -13 * s, -9 * s, -11 * s, -1 * s, -3 * s, -7 * s, -5 * s,
+          15 * s, 13 * s, 9 * s, 11 * s, 1 * s, 3 * s, 7 * s, 5 * s]
+    t = np.zeros(256, dtype=np.complex64)
+    for i in range(256):
+        im_i = ((i & 0x40) >> 3) + ((i & 0x10) >> 2) + ((i & 0x4) >> 1) + (i & 1)
+        re_i = ((i & 0x80) >> 4) + ((i & 0x20) >> 3) + ((i & 0x8) >> 2) + ((i & 0x2) >> 1)
+        t[i] = complex(lv[re_i], lv[im_i])
+    return t
+
+
+def bits_to_index(bits: np.ndarray) -> np.ndarray:
+    """bits[..., Q] (first stream bit first) -> symbol index, MSB-first."""
+    Q = bits.shape[-1]
+    w = (1 << np.arange(Q - 1, -1, -1)).astype(np.int64)
+    return (bits.astype(np.int64) * w).sum(-1)
+
+
+def modulate(bits: np.ndarray) -> np.ndarray:
+    return qam_table(bits.shape[-1])[bits_to_index(bits)]
+
+
+# --------------------------------------------------------------------------
+# channels
+# --------------------------------------------------------------------------
+
+def _cn(rng, shape, var=1.0):
+    s = math.sqrt(var / 2)
+    return (rng.standard_normal(shape, dtype=np.float32) * s
+            + 1j * rng.standard_normal(shape, dtype=np.float32) * s).astype(np.complex64)
+
+
+def channel_iid(rng, G: int, B: int, U: int) -> np.ndarray:
+    """H[G][B][U] i.i.d. CN(0,1)."""
+    return _cn(rng, (G, B, U))
+
+
+def exp_corr(n: int, rho: float) -> np.ndarray:
+    i = np.arange(n)
+    return rho ** np.abs(i[:, None] - i[None, :])
+
+
+def _sqrtm_psd(R: np.ndarray) -> np.ndarray:
+    w, V = np.linalg.eigh(R)
+    return (V * np.sqrt(np.maximum(w, 0))) @ V.conj().T
+
+
+def channel_kronecker(rng, G: int, B: int, U: int, rho_rx: float,
+                      rho_tx: float) -> np.ndarray:
+    """H = R_rx^{1/2} H_w R_tx^{1/2}, [R]_ij = rho^|i-j| (per group)."""
+    Rr = _sqrtm_psd(exp_corr(B, rho_rx)).astype(np.complex64)
+    Rt = _sqrtm_psd(exp_corr(U, rho_tx)).astype(np.complex64)
+    Hw = _cn(rng, (G, B, U))
+    return (Rr[None] @ Hw @ Rt[None]).astype(np.complex64)
+
+
+# --------------------------------------------------------------------------
+# LLR layout (src/config/config.cc:84-135, src/operators/ldpc/config.h:61-72,
+# src/baseband/buffers.cc:97-103 with the 128-byte kCoalesce padding)
+# --------------------------------------------------------------------------
+
+LDPC_ZC = sorted([2, 4, 8, 16, 32, 64, 128, 256, 3, 6, 12, 24, 48, 96, 192,
+                  384, 5, 10, 20, 40, 80, 160, 320, 7, 14, 28, 56, 112, 224, 9,
+                  18, 36, 72, 144, 288, 11, 22, 44, 88, 176, 352, 13, 26, 52,
+                  104, 208, 15, 30, 60, 120, 240])
+
+
+@dataclass(frozen=True)
+class LlrLayout:
+    cb_size: int      # bits of one code block (encoded bits)
+    zeros: int        # punctured zero prefix per code block
+    cb_stride: int    # halves per code-block row (padded to 64 halves)
+    cb_blocks: int    # code blocks per user per symbol
+
+    @property
+    def ue_stride(self) -> int:
+        return self.cb_stride * self.cb_blocks
+
+    @property
+    def valid_bits(self) -> int:
+        return self.cb_size * self.cb_blocks
+
+
+def pad64(n: int) -> int:
+    """Leading dimension of a kCoalesce binary16 tensor: 128-byte multiple."""
+    return (n + 63) // 64 * 64
+
+
+def standalone_layout(N: int, Q: int) -> LlrLayout:
+    """One code block holding every bit of the symbol, no punctured prefix."""
+    return LlrLayout(cb_size=N * Q, zeros=0, cb_stride=pad64(N * Q), cb_blocks=1)
+
+
+def ldpc_uplink_layout(N: int, Q: int, code_rate: float = 1 / 3,
+                       base_graph: int = 1, cb_per_symbol: int = 1) -> LlrLayout:
+    """The layout the reference pipeline hands to equalize: cb_size =
+    encoded_bits, zeros = punctured_bits (config.cc:97-117, ldpc/config.h)."""
+    info = 22 if base_graph == 1 else 10
+    uncoded = int(N * Q * code_rate)
+    zc = LDPC_ZC[-1]
+    for i in range(len(LDPC_ZC) - 1):
+        if info * LDPC_ZC[i] * cb_per_symbol <= uncoded < info * LDPC_ZC[i + 1] * cb_per_symbol:
+            zc = LDPC_ZC[i]
+            break
+    rows = int(round(info / code_rate)) - (info - 2)
+    encoded = (info - 2 + rows) * zc
+    punct = 2 * zc
+    return LlrLayout(cb_size=encoded, zeros=punct, cb_stride=pad64(encoded + punct),
+                     cb_blocks=cb_per_symbol)
+
+
+# --------------------------------------------------------------------------
+# slots
+# --------------------------------------------------------------------------
+
+@dataclass
+class Slot:
+    csi: np.ndarray    # complex64 [G][B][U]  (A = H^H)
+    y: np.ndarray      # complex64 [S][B][N]
+    x: np.ndarray      # complex64 [S][U][N]  transmitted symbols
+    bits: np.ndarray   # uint8 [S][U][N][Q]
+    H: np.ndarray      # complex64 [G][B][U]
+
+
+def make_slot(rng, U: int, B: int, N: int, sg: int, S: int, Q: int,
+              snr_db: float, channel: str = "iid", rho_rx: float = 0.5,
+              rho_tx: float = 0.3, H: np.ndarray | None = None) -> Slot:
+    G = (N + sg - 1) // sg
+    if H is None:
+        if channel == "iid":
+            H = channel_iid(rng, G, B, U)
+        elif channel == "kronecker":
+            H = channel_kronecker(rng, G, B, U, rho_rx, rho_tx)
+        else:
+            raise ValueError(channel)
+    bits = rng.integers(0, 2, size=(S, U, N, Q), dtype=np.uint8)
+    x = modulate(bits).astype(np.complex64)                     # [S][U][N]
+    sigma2 = 10.0 ** (-snr_db / 10.0)
+    gidx = np.arange(N) // sg
+    Hk = H[gidx]                                                # [N][B][U]
+    # y[s, b, k] = sum_u H[g(k), b, u] x[s, u, k]
+    y = np.einsum("kbu,suk->sbk", Hk, x, optimize=True)
+    y = (y + _cn(rng, y.shape, sigma2)).astype(np.complex64)
+    return Slot(csi=np.conj(H).astype(np.complex64), y=y, x=x, bits=bits, H=H)
+
+
+def evm_db(xhat: np.ndarray, x: np.ndarray) -> float:
+    """EVM in dB of equalized symbols against the transmitted ones."""
+    err = np.mean(np.abs(xhat.astype(np.complex128) - x.astype(np.complex128)) ** 2)
+    ref = np.mean(np.abs(x.astype(np.complex128)) ** 2)
+    return 10 * math.log10(err / ref)

@@ -1,0 +1,51 @@
+"""Write tests/golden/reference_fixtures.npz from the data files that the
+reference's own unit tests read.
+
+Sources (binary fixtures next to the reference tests):
+  test/sgemv/csi.data       U=4 x B=8 CSI, column-major complex64
+                            (test/sgemv/test_sgemv.cc:34-46)
+  test/sgemv/bsVec.data     y as [ofdm=16][bs=8] complex64; the test transposes
+                            it to [bs][ofdm] before upload (test_sgemv.cc:136-146)
+  test/sgemv/equalize.data  expected QPSK LLRs, binary16, [ue=4][2*ofdm=32]
+                            (test_sgemv.cc:126-159, compared with EXPECT_EQ)
+  test/beamform/csi_in.data   U=4 x B=8 CSI (test/beamform/test_beamform.cc:19-33)
+  test/beamform/csi_out.data  expected ZF weights, same layout (1e-5, :91)
+  test/beamform/precoder.data expected normalised precoder, B x U (1e-5, :92)
+
+The arrays are stored unchanged (raw little-endian values); this script only
+re-packages them so the tests run where the reference tree is absent.
+Run:  python tests/golden/make_reference_fixtures.py [/root/reference]
+"""
+import os
+import sys
+
+import numpy as np
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                   "reference_fixtures.npz")
+
+
+def rd(rel, dtype):
+    return np.fromfile(os.path.join(REF, rel), dtype=dtype)
+
+
+def main():
+    arrays = {
+        "sgemv_csi": rd("test/sgemv/csi.data", np.complex64),
+        "sgemv_bsvec_kb": rd("test/sgemv/bsVec.data", np.complex64),
+        "sgemv_equalize_f16bits": rd("test/sgemv/equalize.data", np.uint16),
+        "beam_csi_in": rd("test/beamform/csi_in.data", np.complex64),
+        "beam_csi_out": rd("test/beamform/csi_out.data", np.complex64),
+        "beam_precoder": rd("test/beamform/precoder.data", np.complex64),
+    }
+    assert arrays["sgemv_csi"].size == 4 * 8
+    assert arrays["sgemv_bsvec_kb"].size == 16 * 8
+    assert arrays["sgemv_equalize_f16bits"].size == 4 * 32
+    assert arrays["beam_csi_in"].size == 4 * 8
+    np.savez(OUT, **arrays)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

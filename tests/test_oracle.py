@@ -1,0 +1,283 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each oracle function is checked against the reference's own fixtures and
+known-answer tests, against library routines (numpy float16, numpy.linalg), and
+against invariants the mathematics fixes, so that a dropped term, wrong sign or
+transposed operand fails at least one test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from megastation_amd import synth
+
+# ----------------------------------------------------------------------------
+# binary16
+# ----------------------------------------------------------------------------
+
+
+def test_f16_all_patterns_roundtrip(orc):
+    L = orc.lib()
+    bits = np.arange(1 << 16, dtype=np.uint32).astype(np.uint16)
+    vals = bits.view(np.float16).astype(np.float32)
+    for h, v in zip(bits[::7], vals[::7]):
+        if np.isnan(v):
+            continue
+        assert L.orc_f32_to_f16(float(v)) == int(h)
+        assert L.orc_f16_to_f32(int(h)) == v
+
+
+def test_f16_rounding_matches_numpy(orc):
+    rng = np.random.default_rng(1)
+    # magnitudes across the whole binary16 range incl. subnormals and overflow
+    x = (rng.standard_normal(4000) * np.exp2(rng.uniform(-27, 17, 4000))).astype(np.float32)
+    # exact midpoints between adjacent halves (ties to even)
+    h = np.arange(0, 0x7bff, 97, dtype=np.uint16)
+    lo = h.view(np.float16).astype(np.float64)
+    hi = (h + 1).view(np.float16).astype(np.float64)
+    mids = ((lo + hi) / 2).astype(np.float32)
+    for v in np.concatenate([x, mids, -mids, np.float32([65504, 65519.99, 65520, 1e6, -1e6, 0.0, -0.0])]):
+        exp = np.float32(v).astype(np.float16).view(np.uint16)
+        assert orc.f32_to_f16_bits(float(v)) == int(exp), v
+
+
+def test_f16_sub_is_correctly_rounded(orc):
+    rng = np.random.default_rng(2)
+    a = rng.integers(0, 0x7bff, 3000).astype(np.uint16)
+    b = rng.integers(0, 0x7bff, 3000).astype(np.uint16)
+    b[::3] |= 0x8000
+    for x, y in zip(a, b):
+        exact = np.float64(x.view(np.float16)) - np.float64(y.view(np.float16))
+        exp = np.float64(exact).astype(np.float16).view(np.uint16)
+        assert orc.f16_sub_bits(int(x), int(y)) == int(exp)
+
+
+# ----------------------------------------------------------------------------
+# demod
+# ----------------------------------------------------------------------------
+
+
+def _f16(bits):
+    return np.asarray(bits, dtype=np.uint16).view(np.float16).astype(np.float64)
+
+
+@pytest.mark.parametrize("Q", [2, 4, 6, 8])
+def test_demod_hard_decisions_invert_the_generator(orc, Q):
+    """Noise-free constellation points decode to their own bits (LLR>0 <=> 0)."""
+    M = 1 << Q
+    bits = ((np.arange(M)[:, None] >> np.arange(Q - 1, -1, -1)) & 1).astype(np.uint8)
+    pts = synth.modulate(bits)
+    llr = _f16(orc.demod(pts, Q))
+    assert np.array_equal((llr < 0).astype(np.uint8), bits)
+
+
+def test_demod256_reference_table_is_complemented(orc):
+    """Defect D3: the reference 256-QAM table decodes to the complement."""
+    bits = ((np.arange(256)[:, None] >> np.arange(7, -1, -1)) & 1).astype(np.uint8)
+    llr = _f16(orc.demod(synth.reference_table(8), 8))
+    assert np.array_equal((llr < 0).astype(np.uint8), 1 - bits)
+
+
+def test_demod64_known_answer_roundtrip(orc):
+    """test/modulation/test_modulation.cc:11-57: symbol values 0..63 packed
+    LSB-first into bytes, modulated (first stream bit = table MSB,
+    modulation_kernel.cuh:54-58), hard-decided with demod64QAM."""
+    vals = np.arange(64, dtype=np.uint8)
+    stream = np.zeros(64 * 6 // 8, dtype=np.uint8)
+    for bit in range(64 * 6):
+        stream[bit // 8] |= ((vals[bit // 6] >> (bit % 6)) & 1) << (bit % 8)
+    table = synth.qam_table(6)
+    out = []
+    for c in range(64):
+        idx = 0
+        for i in range(6):
+            m = c * 6 + i
+            idx = ((stream[m // 8] >> (m % 8)) & 1) | (idx << 1)
+        llr = _f16(orc.demod(np.complex64(table[idx]), 6))
+        out.append(sum(int(llr[b] < 0) << b for b in range(6)))
+    assert out == list(range(64))
+
+
+def test_demod_closed_forms(orc):
+    f16 = lambda v: np.float16(np.float32(v))
+    # x = 0: the magnitude LLRs equal the thresholds
+    l = _f16(orc.demod(np.complex64(0), 4))
+    t = f16(np.float32(200) / np.sqrt(np.float32(10)))
+    assert list(l) == [0, 0, float(t), float(t)]
+    l = _f16(orc.demod(np.complex64(0), 6))
+    t1 = float(f16(np.float32(400) / np.sqrt(np.float32(42))))
+    t2 = float(f16(np.float32(200) / np.sqrt(np.float32(42))))
+    assert list(l) == [0, 0, t1, t1, float(np.float16(t2 - t1)), float(np.float16(t2 - t1))]
+    l = _f16(orc.demod(np.complex64(0), 8))
+    assert l[2] == float(f16(np.float32(800) / np.sqrt(np.float32(170))))
+    # QPSK is linear with slope -20*sqrt(2), sign-inverted relative to 16-QAM
+    for v in [0.5, -0.25, 1.75]:
+        l = _f16(orc.demod(np.complex64(complex(v, -v)), 2))
+        e = float(np.float16(np.float32(v) * np.float32(-20) * np.sqrt(np.float32(2))))
+        assert l[0] == e and l[1] == -e
+    # 16-QAM: at the outer points the amplitude LLR is -T/... piecewise linear
+    s = 1 / math.sqrt(10)
+    l = _f16(orc.demod(np.complex64(complex(3 * s, -s)), 4))
+    assert l[0] > 0 and l[1] < 0 and l[2] < 0 and l[3] > 0
+    assert abs(l[2] + l[3]) < 0.2  # |T - 300/sqrt10| == |T - 100/sqrt10|
+
+
+# ----------------------------------------------------------------------------
+# equalize
+# ----------------------------------------------------------------------------
+
+
+def _fixture_problem(golden):
+    U, B, N = 4, 8, 16
+    csi = golden["sgemv_csi"]
+    y = golden["sgemv_bsvec_kb"].reshape(N, B).T.copy()   # [B][N], as the test uploads
+    return U, B, N, csi, y
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_equalize_reference_fixture_bit_exact(orc, golden, precision):
+    """test/sgemv/test_sgemv.cc:126-159: equalize(csi, bsVec, ueVec, 16, 2,
+    2*ofdm, 0, 1) compared with EXPECT_EQ on every half."""
+    U, B, N, csi, y = _fixture_problem(golden)
+    llr, _ = orc.equalize(csi, y, U, B, N, sg=16, Q=2, ue_stride=32, cb_size=32,
+                          cb_stride=32, zeros=0, valid_bits=32, precision=precision)
+    assert np.array_equal(llr.reshape(-1), golden["sgemv_equalize_f16bits"])
+
+
+def test_equalize_brute_force(orc):
+    rng = np.random.default_rng(5)
+    U, B, N, sg, Q = 5, 12, 50, 8, 6      # partial last group (50 % 8 != 0)
+    G = (N + sg - 1) // sg
+    csi = synth._cn(rng, (G, B, U))
+    y = synth._cn(rng, (B, N))
+    lay = synth.standalone_layout(N, Q)
+    _, x32 = orc.equalize(csi, y, U, B, N, sg, Q, lay.ue_stride, lay.cb_size,
+                          lay.cb_stride, lay.zeros, lay.valid_bits, "f32")
+    _, x64 = orc.equalize(csi, y, U, B, N, sg, Q, lay.ue_stride, lay.cb_size,
+                          lay.cb_stride, lay.zeros, lay.valid_bits, "f64")
+    W = csi.astype(np.complex128).transpose(0, 2, 1)      # [G][U][B]
+    ref = np.stack([W[k // sg] @ y[:, k].astype(np.complex128) for k in range(N)], 1)
+    assert np.allclose(x64, ref, rtol=0, atol=1e-12)
+    assert np.abs(x32 - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("Q", [2, 4, 6, 8])
+def test_equalize_llr_placement(orc, Q):
+    """Each user row: code block c occupies [c*cb_stride + zeros,
+    c*cb_stride + zeros + cb_size); bits beyond valid_bits, the punctured
+    prefix and the padding are never written (sentinel stays)."""
+    rng = np.random.default_rng(7)
+    U, B, N, sg = 3, 4, 40, 4
+    G = N // sg
+    csi = synth._cn(rng, (G, B, U))
+    y = synth._cn(rng, (B, N))
+    cb_size, zeros, blocks = 50, 6, 3          # N*Q > cb_size: spans blocks
+    cb_stride = synth.pad64(cb_size + zeros)
+    ue_stride = cb_stride * blocks
+    valid = min(cb_size * blocks, N * Q)
+    sentinel = np.full((U, ue_stride), 0x7e01, dtype=np.uint16)
+    llr, x = orc.equalize(csi, y, U, B, N, sg, Q, ue_stride, cb_size, cb_stride,
+                          zeros, cb_size * blocks, "f32", llr_init=sentinel)
+    d = orc.demod(x, Q).reshape(U, N * Q)                 # stream of bits per user
+    for u in range(U):
+        written = np.flatnonzero(llr[u] != 0x7e01)
+        # every stream bit i < valid lands at block(i) ... except those of a
+        # symbol that straddles a block boundary, which continue past cb_size
+        expect = {}
+        for k in range(N):
+            i0 = k * Q
+            base = (i0 // cb_size) * cb_stride + zeros + i0 % cb_size
+            for j in range(Q):
+                if i0 + j < cb_size * blocks and base + j < ue_stride:
+                    expect[base + j] = d[u, i0 + j]
+        assert sorted(expect) == list(written)
+        assert all(llr[u, p] == v for p, v in expect.items())
+        for c in range(blocks):
+            assert np.all(llr[u, c * cb_stride: c * cb_stride + zeros] == 0x7e01)
+        assert len(written) >= valid - Q
+
+
+# ----------------------------------------------------------------------------
+# zero forcing
+# ----------------------------------------------------------------------------
+
+
+def test_zf_reference_fixture(orc, golden):
+    """test/beamform/test_beamform.cc:67-95: batch of 2 copies, 1e-5."""
+    a = np.concatenate([golden["beam_csi_in"]] * 2)
+    for prec in ["f64", "f32"]:
+        w, _, info = orc.zf(a, 4, 8, 2, prec)
+        assert not info.any()
+        for g in range(2):
+            assert np.abs(w[32 * g: 32 * g + 32] - golden["beam_csi_out"]).max() < 1e-5
+
+
+def test_precoder_reference_fixture(orc, golden):
+    w = np.concatenate([golden["beam_csi_out"]] * 2)
+    p = orc.scale_absmax(w, 4, 8, 2)
+    for g in range(2):
+        assert np.abs(p[32 * g: 32 * g + 32] - golden["beam_precoder"]).max() < 1e-6
+
+
+def test_zf_matches_pinv_and_invariants(orc):
+    rng = np.random.default_rng(11)
+    U, B, G = 6, 20, 7
+    H = synth.channel_kronecker(rng, G, B, U, 0.5, 0.3)
+    a = np.conj(H)                                   # [G][B][U] = A^H per group
+    w, w64, info = orc.zf(a, U, B, G, "f64")
+    assert not info.any()
+    W = w64.reshape(G, B, U).transpose(0, 2, 1)      # [G][U][B]
+    A = a.astype(np.complex128).transpose(0, 2, 1)   # [G][U][B]
+    for g in range(G):
+        Hg = H[g].astype(np.complex128)              # B x U
+        assert np.allclose(W[g], np.linalg.pinv(Hg), atol=1e-10)
+        assert np.allclose(W[g] @ A[g].conj().T, np.eye(U), atol=1e-10)   # W A^H = I
+    # ZF is an involution: ZF(ZF(A)) = A
+    back, _, _ = orc.zf(w64.astype(np.complex64), U, B, G, "f64")
+    assert np.abs(back - a.reshape(-1)).max() < 1e-4 * np.abs(a).max()
+    w32, _, _ = orc.zf(a, U, B, G, "f32")
+    assert np.abs(w32 - w).max() < 1e-4 * np.abs(w).max()
+
+
+def test_zf_rank_deficient_reports_info(orc):
+    rng = np.random.default_rng(3)
+    U, B = 4, 8
+    H = synth.channel_iid(rng, 2, B, U)
+    H[1, :, 2] = H[1, :, 0]                          # duplicate user in group 1
+    w, _, info = orc.zf(np.conj(H), U, B, 2, "f64")
+    assert info[0] == 0 and info[1] == 3
+    assert not np.any(w.reshape(2, -1)[1])
+
+
+# ----------------------------------------------------------------------------
+# CPU baseline == oracle composition
+# ----------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("Q,N,sg", [(4, 96, 16), (6, 100, 12)])
+def test_detect_cpu_equals_zf_then_equalize(orc, Q, N, sg):
+    rng = np.random.default_rng(13)
+    U, B, S, slots = 4, 16, 3, 2
+    lay = synth.ldpc_uplink_layout(N, Q) if Q == 6 else synth.standalone_layout(N, Q)
+    sl = [synth.make_slot(rng, U, B, N, sg, S, Q, 20.0) for _ in range(slots)]
+    csi = np.stack([s.csi for s in sl])
+    y = np.stack([s.y for s in sl])
+    llr, nt = orc.detect_cpu(csi, y, slots, S, U, B, N, sg, Q, lay.ue_stride,
+                             lay.cb_size, lay.cb_stride, lay.zeros, lay.valid_bits)
+    assert nt >= 1
+    G = (N + sg - 1) // sg
+    for s in range(slots):
+        w, _, _ = orc.zf(csi[s], U, B, G, "f32")
+        for t in range(S):
+            ref, _ = orc.equalize(w, y[s, t], U, B, N, sg, Q, lay.ue_stride,
+                                  lay.cb_size, lay.cb_stride, lay.zeros,
+                                  lay.valid_bits, "f32")
+            assert np.array_equal(llr[s, t], ref)
+
+
+def test_ldpc_layout_matches_reference_sizing():
+    """config.cc:84-135 with rate 1/3, BG1 (main.cc:71-72)."""
+    lay = synth.ldpc_uplink_layout(3276, 6)
+    assert (lay.cb_size, lay.zeros, lay.cb_stride) == (66 * 288, 2 * 288, 19584)
+    assert synth.pad64(5) == 64 and synth.pad64(64) == 64   # test_cuphy_tensor.cc:29-42
