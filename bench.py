@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Detected symbols/s of the uplink detection path (ZF + equalize + demod) on
+one MI355X.
+
+Workload (default): a 64-antenna x 16-user cell, 1200 data subcarriers in
+groups of 16 (75 groups), 13 data OFDM symbols per slot, 16-QAM at 20 dB SNR
+over i.i.d. CN(0,1) channels, batched over 200 slots. One step = ZF weights of
+all 200 x 75 groups (one launch) + equalize/demod of all 200 x 13 symbols (one
+launch), inputs resident in HBM. Synthetic data generated on the GPU with a
+seeded Philox generator.
+
+Prints one JSON line: value = detected symbols/s = U*N*S*slots / step time;
+roofline of the dominant kernel (equalize/demod, HBM-bound) from HIP events
+on the launch stream; the CPU baseline (the oracle's OpenMP restatement of
+the same operator math) timed on a bounded sample on the host cores; EVM of
+the GPU's equalized symbols against the FP64 oracle's on a sample slot.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "detected symbols/s (ZF+equalize+demod), 1 GPU, at EVM within 0.1 dB of FP64"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # 64x16 cell x 200 slots (the batched GPU configuration)
+    "c2_64x16_200slots": dict(U=16, B=64, N=1200, sg=16, S=13, slots=200, Q=4,
+                              snr=20.0, channel="iid"),
+    # 256x32 cell, one slot
+    "c3_256x32": dict(U=32, B=256, N=3276, sg=12, S=13, slots=1, Q=6, snr=25.0,
+                      channel="kronecker"),
+}
+
+
+def gen_inputs(cfg, seed=1234):
+    """CSI [slots][G][B][U] (= H^H) and y [slots][S][B][N] on cuda:0."""
+    import torch
+    from megastation_amd import synth
+    U, B, N, sg, S, slots, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "slots", "Q"))
+    G = (N + sg - 1) // sg
+    Np = G * sg
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    if cfg["channel"] == "iid":
+        H = torch.randn((slots * G, B, U), dtype=torch.complex64, device=dev, generator=gen)
+    else:
+        rng = np.random.Generator(np.random.Philox(seed))
+        Hn = np.stack([synth.channel_kronecker(rng, G, B, U, 0.5, 0.3) for _ in range(slots)])
+        H = torch.from_numpy(Hn.reshape(slots * G, B, U)).to(dev)
+    table = torch.from_numpy(synth.qam_table(Q)).to(dev)
+    idx = torch.randint(0, 1 << Q, (slots, S, U, Np), device=dev, generator=gen)
+    x = table[idx]                                             # [slots,S,U,Np]
+    X = x.reshape(slots, S, U, G, sg).permute(0, 3, 2, 1, 4).reshape(slots * G, U, S * sg)
+    Hr, Hi = H.real.contiguous(), H.imag.contiguous()
+    Xr, Xi = X.real.contiguous(), X.imag.contiguous()
+    Yr = torch.bmm(Hr, Xr) - torch.bmm(Hi, Xi)
+    Yi = torch.bmm(Hr, Xi) + torch.bmm(Hi, Xr)
+    del Xr, Xi
+    Y = torch.complex(Yr, Yi)
+    del Yr, Yi
+    sigma = math.sqrt(10.0 ** (-cfg["snr"] / 10.0))
+    Y = Y + sigma * torch.randn(Y.shape, dtype=torch.complex64, device=dev, generator=gen)
+    y = (Y.reshape(slots, G, B, S, sg).permute(0, 3, 2, 1, 4)
+         .reshape(slots, S, B, Np)[..., :N].contiguous())
+    csi = torch.conj(H).resolve_conj().reshape(slots, G, B, U).contiguous()
+    x = x[..., :N].contiguous()
+    return csi, y, x
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2_64x16_200slots", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="approximate CPU time budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the detection path shards by subcarrier group with no exchange; "
+              "run one process per GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import megastation_amd as m
+    from megastation_amd import synth
+
+    cfg = WORKLOADS[args.workload]
+    U, B, N, sg, S, slots, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "slots", "Q"))
+    G = (N + sg - 1) // sg
+    lay = synth.standalone_layout(N, Q)
+    dev = torch.device("cuda:0")
+    print(f"[bench] workload {args.workload}: generating inputs", file=sys.stderr, flush=True)
+    csi, y, x = gen_inputs(cfg)
+    w = torch.empty_like(csi)
+    llr = torch.empty(slots * S * U * lay.ue_stride, dtype=torch.int16, device=dev)
+    stream = torch.cuda.current_stream()
+
+    eq_kw = dict(num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S,
+                 num_slots=slots, mod_order=Q, ue_stride=lay.ue_stride,
+                 cb_size=lay.cb_size, cb_stride=lay.cb_stride, zeros=lay.zeros,
+                 valid_bits=lay.valid_bits, csi_slot_stride=G * B * U,
+                 y_sym_stride=B * N, y_slot_stride=S * B * N,
+                 llr_sym_stride=U * lay.ue_stride, llr_slot_stride=S * U * lay.ue_stride)
+
+    def step(evs=None):
+        if evs:
+            evs[0].record(stream)
+        m.zf(csi, w, B, U, slots * G, stream=stream)
+        if evs:
+            evs[1].record(stream)
+        m.equalize_slots(w, y, llr, stream=stream, **eq_kw)
+        if evs:
+            evs[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for i in range(args.steps):
+        step(evs[i])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t0.elapsed_time(t1)
+    ms_step = total_ms / args.steps
+    zf_ms = [e[0].elapsed_time(e[1]) for e in evs]
+    eq_ms = [e[1].elapsed_time(e[2]) for e in evs]
+    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    symbols = U * N * S * slots
+    value = symbols / (ms_step * 1e-3)
+    eq_bytes = 8 * B * N * S * slots + 8 * U * B * G * slots + 2 * U * Q * N * S * slots
+    eq_avg = statistics.mean(eq_ms)
+    achieved = eq_bytes / (eq_avg * 1e-3) / 1e9
+    zf_flops = G * slots * (4 * U * (U + 1) * B + 8 * U * U * B + (4 / 3) * U ** 3)
+
+    # ---- EVM of the GPU path on slot 0 (x_hat side output, untimed) --------
+    xh = torch.zeros(S * U * N, dtype=torch.complex64, device=dev)
+    kw1 = dict(eq_kw, num_slots=1)
+    llr1 = torch.empty(S * U * lay.ue_stride, dtype=torch.int16, device=dev)
+    m.equalize_slots(w[0], y[0], llr1, xhat=xh, stream=stream, **kw1)
+    torch.cuda.synchronize()
+    xh_h = xh.cpu().numpy().reshape(S, U, N)
+    x0 = x[0].cpu().numpy()
+    evm_gpu = synth.evm_db(xh_h, x0)
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "symbols/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": args.workload, "antennas": B, "users": U,
+                   "subcarriers": N, "sc_group": sg, "groups": G, "data_symbols": S,
+                   "slots": slots, "qam": 1 << Q, "snr_db": cfg["snr"],
+                   "channel": cfg["channel"], "form": "fp32",
+                   "parallelism": "1 GPU, groups x slots"},
+        "roofline": {"bound": "hbm", "kernel": "equalize_demod",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": eq_bytes, "avg_ms": eq_avg},
+        "kernels": {"zf_ms": statistics.mean(zf_ms), "equalize_ms": eq_avg,
+                    "zf_tflops": zf_flops / (statistics.mean(zf_ms) * 1e-3) / 1e12},
+        "evm": {"gpu_db": evm_gpu},
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"], evm64 = cpu_baseline(cfg, lay, csi, y, x, xh_h,
+                                                     args.cpu_seconds)
+        result["evm"]["fp64_db"] = evm64
+        result["evm"]["delta_db"] = evm_gpu - evm64
+        result["evm"]["within_0p1_db"] = abs(evm_gpu - evm64) < 0.1
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(cfg, lay, csi, y, x, xh_gpu, budget_s):
+    """The oracle's OpenMP restatement (FP32 Gram -> Cholesky -> W, then
+    equalize + demod per group) timed on the host cores over a bounded sample
+    of the same slots; plus the FP64 oracle's EVM on slot 0."""
+    import oracle
+    from megastation_amd import synth
+    if not os.path.exists(oracle.LIB_PATH):
+        oracle.build()
+    U, B, N, sg, S, slots, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "slots", "Q"))
+    G = (N + sg - 1) // sg
+    threads = len(os.sched_getaffinity(0))
+    n = 1
+    c_h = csi[:n].cpu().numpy()
+    y_h = y[:n].cpu().numpy()
+    t = time.perf_counter()
+    oracle.detect_cpu(c_h, y_h, n, S, U, B, N, sg, Q, lay.ue_stride, lay.cb_size,
+                      lay.cb_stride, lay.zeros, lay.valid_bits, threads)
+    one = time.perf_counter() - t
+    n = int(max(1, min(slots, budget_s / 6 / max(one, 1e-6))))
+    c_h = csi[:n].cpu().numpy()
+    y_h = y[:n].cpu().numpy()
+    llr_h = np.zeros((n, S, U, lay.ue_stride), dtype=np.uint16)
+    oracle.detect_cpu(c_h, y_h, n, S, U, B, N, sg, Q, lay.ue_stride, lay.cb_size,
+                      lay.cb_stride, lay.zeros, lay.valid_bits, threads, llr=llr_h)
+    times = []
+    for _ in range(5):
+        t = time.perf_counter()
+        _, nt = oracle.detect_cpu(c_h, y_h, n, S, U, B, N, sg, Q, lay.ue_stride,
+                                  lay.cb_size, lay.cb_stride, lay.zeros,
+                                  lay.valid_bits, threads, llr=llr_h)
+        times.append(time.perf_counter() - t)
+    med = statistics.median(times)
+    base = {"value": U * N * S * n / med, "unit": "symbols/s", "cores": nt,
+            "kind": "port",
+            "sample": f"{n} of {slots} slots ({U * N * S * n} symbols), median of 5 "
+                      f"runs after 1 warm-up, OpenMP over groups, {nt} threads"}
+    # FP64 judge on slot 0
+    w64, _, _ = oracle.zf(csi[0].cpu().numpy(), U, B, G, "f64")
+    y0 = y[0].cpu().numpy()
+    x64 = np.stack([oracle.equalize(w64, y0[s], U, B, N, sg, Q, lay.ue_stride,
+                                    lay.cb_size, lay.cb_stride, lay.zeros,
+                                    lay.valid_bits, "f64")[1] for s in range(S)])
+    evm64 = synth.evm_db(x64, x[0].cpu().numpy())
+    return base, evm64
+
+
+if __name__ == "__main__":
+    main()

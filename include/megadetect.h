@@ -1,0 +1,152 @@
+/*
+ * megadetect.h — C ABI of the MI355X uplink detection path (ZF beamforming
+ * weights + fused equalize / max-log demod), libmegadetect.so.
+ *
+ * Plain C: fixed-width integers, raw device pointers, no exceptions. Every
+ * call runs asynchronously on the given stream of the current HIP device
+ * (stream = a hipStream_t; NULL is the device's null stream) unless it says it
+ * synchronises. Functions return MEGA_OK (0) or a negative MEGA_ERR_* code;
+ * mega_last_error() then holds a message for the calling thread.
+ *
+ * Buffer layouts are the reference's (src/operators/mega_complex.h:25-27,
+ * src/operators/matrix/matrix.h:95-117, src/baseband/buffers.cc:77-103):
+ *   complex      interleaved {float re, float im} (complex64)
+ *   CSI / W      [G][B][U]: element (u,b,g) at u + U*b + U*B*g, i.e. one
+ *                column-major U x B matrix A_g = H_g^H per subcarrier group
+ *   precoder     [G][U][B]: one column-major B x U matrix per group
+ *   y (bsVec)    [B][N] per OFDM symbol, subcarrier fastest
+ *   LLR (ueVec)  binary16, one row of ue_stride halves per user per symbol;
+ *                bit j of subcarrier k at
+ *                (kQ / cb_size)*cb_stride + zeros + (kQ % cb_size) + j,
+ *                written when kQ + j < valid_bits and the offset < ue_stride.
+ * Subcarrier k uses group k / sg; a partial last group (N % sg != 0) is
+ * allowed, and the CSI buffer then holds ceil(N / sg) groups.
+ */
+#ifndef MEGADETECT_H
+#define MEGADETECT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MEGA_OK 0
+#define MEGA_ERR_INVALID_ARG (-1)  /* null pointer, zero or mismatched size */
+#define MEGA_ERR_UNSUPPORTED (-2)  /* modulation order, shape beyond limits */
+#define MEGA_ERR_BATCH (-3)        /* batch exceeds the handle's batch_count */
+#define MEGA_ERR_HIP (-4)          /* a HIP runtime call failed */
+
+/* Numerical form of the ZF / equalize kernels. */
+#define MEGA_FORM_FP32 0   /* plain FP32 arithmetic: the drop-in form */
+
+typedef void *mega_stream_t; /* hipStream_t */
+typedef struct mega_beamform *mega_beamform_t;
+
+/* Library version string and the calling thread's last error message. */
+const char *mega_version(void);
+const char *mega_last_error(void);
+
+/* ---- Beamform ------------------------------------------------------------ */
+
+/* Replaces Beamform::Beamform(num_bs, num_ue, batch_count)
+ * (src/operators/beamform/beamform.cu:19-31). Allocates the per-group status
+ * array on the current device. */
+int mega_beamform_create(uint32_t num_bs, uint32_t num_ue, uint32_t batch_count,
+                         mega_beamform_t *out);
+
+/* Replaces Beamform::~Beamform (beamform.cu:33-48). NULL is a no-op. */
+int mega_beamform_destroy(mega_beamform_t bf);
+
+/* Replaces Beamform::beamform_uplink(const Matrix& csi, stream)
+ * (beamform.cu:144-152) and beamformer_uplink (:74-122): in place,
+ * W_g = (A_g A_g^H)^-1 A_g for g < batch. batch > batch_count is
+ * MEGA_ERR_BATCH (the reference throws std::runtime_error). A group whose
+ * Gram matrix is not positive definite gets W_g = 0 and a nonzero status
+ * (mega_beamform_info); the reference never reads its potrf info. */
+int mega_beamform_uplink(mega_beamform_t bf, void *csi, uint32_t batch,
+                         mega_stream_t stream);
+
+/* Replaces Beamform::beamformer(const Matrix& csi, const Matrix& precode,
+ * stream) (beamform.cu:124-142): beamform_uplink, then precode_g =
+ * W_g^T / max|W_g| (beam_utils.cuh:40-80). */
+int mega_beamformer(mega_beamform_t bf, void *csi, void *precode,
+                    uint32_t batch, mega_stream_t stream);
+
+/* Replaces static Beamform::beamform_downlink(csi, precode, stream)
+ * (beamform.cu:154-168): precode_g = csi_g^T / max|csi_g| only. */
+int mega_beamform_downlink(const void *csi, void *precode, uint32_t num_ue,
+                           uint32_t num_bs, uint32_t batch,
+                           mega_stream_t stream);
+
+/* Per-group status of the last beamform_uplink / beamformer on this handle:
+ * 0 = ok, k+1 = Cholesky pivot k was not positive. Synchronises `stream`,
+ * copies batch values to host_info and returns the number of failed groups
+ * (>= 0) or an error code. No reference counterpart (the reference leaves
+ * cuSOLVER's info unread, beamform.cu:110-111). */
+int mega_beamform_info(mega_beamform_t bf, int32_t *host_info, uint32_t batch,
+                       mega_stream_t stream);
+
+/* Out-of-place ZF without a handle: w_out may equal csi_in. info_dev
+ * (optional, device, int32[batch]) receives the per-group status. Used by
+ * callers that keep the channel estimate (the beamformer's in-place contract
+ * overwrites it). */
+int mega_zf(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
+            const void *csi_in, void *w_out, int32_t *info_dev, int form,
+            mega_stream_t stream);
+
+/* ---- Equalize + demod -------------------------------------------------- */
+
+/* Replaces the raw overload Equalize::equalize(problem_size_row = U,
+ * problem_size_col = B, batch_count = N, bCSI, bBsVec, bUeVec, ue_stride,
+ * csi_skip = sg, mod_order = Q, cb_size, cb_stride, zeros, valid_bits, stream)
+ * (src/operators/sgemv/equalize.cu:20-51): one OFDM symbol,
+ * x_k = W_{k/sg} y_k, then max-log demod to binary16 LLRs
+ * (demodulation.h:19-67). Q outside {2,4,6,8} is MEGA_ERR_UNSUPPORTED and
+ * writes nothing (the reference silently writes nothing). The Matrix overload
+ * (equalize.cu:53-67) maps to this with cb_stride = ueVec.stride(1),
+ * ue_stride = cb_stride * cb_blocks, valid_bits = cb_size * cb_blocks. */
+int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
+                  const void *csi, const void *bsvec, void *uevec,
+                  uint64_t ue_stride, uint64_t csi_skip, uint8_t mod_order,
+                  uint64_t cb_size, uint64_t cb_stride, uint64_t zeros,
+                  uint64_t valid_bits, mega_stream_t stream);
+
+/* Many OFDM symbols of many slots in one launch: W of a group is staged once
+ * and applied to every symbol. Element strides (complex elements for csi / y /
+ * xhat, halves for llr):
+ *   csi  slot s at csi + s*csi_slot_stride            ([G][B][U] each)
+ *   y    symbol t of slot s at y + s*y_slot_stride + t*y_sym_stride ([B][N])
+ *   llr  at llr + s*llr_slot_stride + t*llr_sym_stride ([U][ue_stride])
+ *   xhat optional debug output, complex64 [slots][syms][U][N] (NULL = none).
+ * Extends equalize.cu:20-51, which handles one symbol per launch. */
+typedef struct {
+  uint32_t num_ue, num_bs, num_sc, sc_group; /* U, B, N, sg */
+  uint32_t num_syms, num_slots;
+  uint32_t mod_order;                        /* Q */
+  int32_t form;                              /* MEGA_FORM_* */
+  uint64_t ue_stride, cb_size, cb_stride, zeros, valid_bits;
+  const void *csi;
+  uint64_t csi_slot_stride;
+  const void *y;
+  uint64_t y_sym_stride, y_slot_stride;
+  void *llr;
+  uint64_t llr_sym_stride, llr_slot_stride;
+  void *xhat;
+} mega_equalize_desc;
+
+int mega_equalize_slots(const mega_equalize_desc *desc, mega_stream_t stream);
+
+/* ---- host demodulation ----------------------------------------------------
+ * Replaces the host side of demodQPSK / demod16QAM / demod64QAM /
+ * demod256QAM (demodulation.h:30-67, __device__ __host__): n complex64
+ * values -> n*Q binary16 LLRs (uint16 bit patterns), bit-identical to the
+ * device path. */
+int mega_demod_host(const float *xhat_c64, uint64_t n, uint8_t mod_order,
+                    uint16_t *llr_f16);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEGADETECT_H */

@@ -1,0 +1,238 @@
+"""megastation_amd — MI355X (gfx950) uplink detection path of MegaStation.
+
+Zero-forcing beamforming weights per subcarrier group and the fused
+equalize + max-log demod GEMV, as hand-written HIP kernels behind the C ABI of
+``lib/libmegadetect.so`` (declared in ``include/megadetect.h``).
+
+This module is the Python mirror of the reference operator API
+(``src/operators/beamform/beamform.h:27-84``,
+``src/operators/sgemv/equalize.h:25-74``): ``Beamform`` with
+``beamform_uplink`` / ``beamformer`` / ``beamform_downlink`` and
+``Equalize.equalize`` with the reference's argument meaning. Device buffers are
+torch tensors (plumbing for memory and streams); every computation runs in the
+HIP library, and importing the library fails loudly when it is not built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmegadetect.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "megadetect.h")
+
+MEGA_OK = 0
+MEGA_ERR_INVALID_ARG = -1
+MEGA_ERR_UNSUPPORTED = -2
+MEGA_ERR_BATCH = -3
+MEGA_ERR_HIP = -4
+
+FORM_FP32 = 0
+
+_lib = None
+
+
+class MegaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class EqualizeDesc(ctypes.Structure):
+    """Mirror of ``mega_equalize_desc`` (include/megadetect.h)."""
+    _fields_ = [
+        ("num_ue", ctypes.c_uint32), ("num_bs", ctypes.c_uint32),
+        ("num_sc", ctypes.c_uint32), ("sc_group", ctypes.c_uint32),
+        ("num_syms", ctypes.c_uint32), ("num_slots", ctypes.c_uint32),
+        ("mod_order", ctypes.c_uint32), ("form", ctypes.c_int32),
+        ("ue_stride", ctypes.c_uint64), ("cb_size", ctypes.c_uint64),
+        ("cb_stride", ctypes.c_uint64), ("zeros", ctypes.c_uint64),
+        ("valid_bits", ctypes.c_uint64),
+        ("csi", ctypes.c_void_p), ("csi_slot_stride", ctypes.c_uint64),
+        ("y", ctypes.c_void_p), ("y_sym_stride", ctypes.c_uint64),
+        ("y_slot_stride", ctypes.c_uint64),
+        ("llr", ctypes.c_void_p), ("llr_sym_stride", ctypes.c_uint64),
+        ("llr_slot_stride", ctypes.c_uint64),
+        ("xhat", ctypes.c_void_p),
+    ]
+
+
+def declared_symbols() -> list[str]:
+    """Names of the functions include/megadetect.h declares."""
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mega_\w+)\s*\(",
+                                 text, re.M)))
+
+
+def lib() -> ctypes.CDLL:
+    """Load libmegadetect.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m megastation_amd.build`"
+                           " (the detection path has no fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, u8, u32, u64, i32 = (ctypes.c_void_p, ctypes.c_uint8, ctypes.c_uint32,
+                            ctypes.c_uint64, ctypes.c_int)
+    sig = {
+        "mega_version": ([], ctypes.c_char_p),
+        "mega_last_error": ([], ctypes.c_char_p),
+        "mega_beamform_create": ([u32, u32, u32, ctypes.POINTER(P)], i32),
+        "mega_beamform_destroy": ([P], i32),
+        "mega_beamform_uplink": ([P, P, u32, P], i32),
+        "mega_beamformer": ([P, P, P, u32, P], i32),
+        "mega_beamform_downlink": ([P, P, u32, u32, u32, P], i32),
+        "mega_beamform_info": ([P, P, u32, P], i32),
+        "mega_zf": ([u32, u32, u32, P, P, P, i32, P], i32),
+        "mega_equalize": ([u64, u64, u64, P, P, P, u64, u64, u8, u64, u64, u64,
+                           u64, P], i32),
+        "mega_equalize_slots": ([ctypes.POINTER(EqualizeDesc), P], i32),
+        "mega_demod_host": ([P, u64, u8, P], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise MegaError(rc, lib().mega_last_error().decode())
+    return rc
+
+
+def last_error() -> str:
+    return lib().mega_last_error().decode()
+
+
+def _stream(stream):
+    """hipStream_t handle of a torch stream (default: the current stream)."""
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(getattr(stream, "cuda_stream", stream) or 0)
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    if t is None:
+        return ctypes.c_void_p(0)
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------------------
+# Beamform (src/operators/beamform/beamform.h:27-84)
+# ---------------------------------------------------------------------------
+
+class Beamform:
+    """ZF beamformer for ``batch_count`` subcarrier groups of a
+    ``num_bs`` x ``num_ue`` cell. Buffers are complex64 device tensors in the
+    ``[G][B][U]`` CSI layout (precoder ``[G][U][B]``)."""
+
+    def __init__(self, num_bs: int, num_ue: int, batch_count: int):
+        self.num_bs, self.num_ue, self.batch_count = num_bs, num_ue, batch_count
+        h = ctypes.c_void_p()
+        _check(lib().mega_beamform_create(num_bs, num_ue, batch_count, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mega_beamform_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def beamform_uplink(self, csi, batch: int | None = None, stream=None):
+        """In place: csi_g <- (A_g A_g^H)^-1 A_g."""
+        batch = csi.shape[0] if batch is None else batch
+        _check(lib().mega_beamform_uplink(self._h, _ptr(csi), batch, _stream(stream)))
+
+    def beamformer(self, csi, precode, batch: int | None = None, stream=None):
+        batch = csi.shape[0] if batch is None else batch
+        _check(lib().mega_beamformer(self._h, _ptr(csi), _ptr(precode), batch,
+                                     _stream(stream)))
+
+    def info(self, batch: int | None = None, stream=None):
+        """Per-group Cholesky status of the last call (synchronises)."""
+        import numpy as np
+        batch = self.batch_count if batch is None else batch
+        out = np.zeros(batch, dtype=np.int32)
+        _check(lib().mega_beamform_info(self._h, out.ctypes.data_as(ctypes.c_void_p),
+                                        batch, _stream(stream)))
+        return out
+
+    @staticmethod
+    def beamform_downlink(csi, precode, num_ue: int, num_bs: int, batch: int,
+                          stream=None):
+        _check(lib().mega_beamform_downlink(_ptr(csi), _ptr(precode), num_ue,
+                                            num_bs, batch, _stream(stream)))
+
+
+def zf(csi_in, w_out, num_bs: int, num_ue: int, batch: int, info=None,
+       form: int = FORM_FP32, stream=None):
+    """Out-of-place ZF (w_out may be csi_in)."""
+    _check(lib().mega_zf(num_bs, num_ue, batch, _ptr(csi_in), _ptr(w_out),
+                         _ptr(info), form, _stream(stream)))
+
+
+# ---------------------------------------------------------------------------
+# Equalize (src/operators/sgemv/equalize.h:25-74)
+# ---------------------------------------------------------------------------
+
+class Equalize:
+    @staticmethod
+    def equalize_raw(row, col, batch_count, csi, bsvec, uevec, ue_stride,
+                     csi_skip, mod_order, cb_size, cb_stride, zeros, valid_bits,
+                     stream=None):
+        """equalize.cu:20-51 argument order: U, B, N, W, y, llr, ..."""
+        _check(lib().mega_equalize(row, col, batch_count, _ptr(csi), _ptr(bsvec),
+                                   _ptr(uevec), ue_stride, csi_skip, mod_order,
+                                   cb_size, cb_stride, zeros, valid_bits,
+                                   _stream(stream)))
+
+    @staticmethod
+    def equalize(csi, bsvec, uevec, csi_skip, mod_order, cb_size, zeros,
+                 cb_blocks, stream=None):
+        """Matrix overload (equalize.cu:53-67) for torch tensors:
+        csi [G][B][U] complex64, bsvec [B][N] complex64, uevec
+        [U*cb_blocks][cb_stride] float16 (cb_stride = uevec row stride)."""
+        U, B = csi.shape[-1], csi.shape[-2]
+        N = bsvec.shape[-1]
+        cb_stride = uevec.stride(0)
+        ue_stride = cb_stride * cb_blocks
+        Equalize.equalize_raw(U, B, N, csi, bsvec, uevec, ue_stride, csi_skip,
+                              mod_order, cb_size, cb_stride, zeros,
+                              cb_size * cb_blocks, stream)
+
+
+def equalize_slots(csi, y, llr, *, num_ue, num_bs, num_sc, sc_group, num_syms,
+                   num_slots, mod_order, ue_stride, cb_size, cb_stride, zeros,
+                   valid_bits, csi_slot_stride, y_sym_stride, y_slot_stride,
+                   llr_sym_stride, llr_slot_stride, xhat=None,
+                   form: int = FORM_FP32, stream=None):
+    d = EqualizeDesc(num_ue, num_bs, num_sc, sc_group, num_syms, num_slots,
+                     mod_order, form, ue_stride, cb_size, cb_stride, zeros,
+                     valid_bits, _ptr(csi).value, csi_slot_stride, _ptr(y).value,
+                     y_sym_stride, y_slot_stride, _ptr(llr).value, llr_sym_stride,
+                     llr_slot_stride, _ptr(xhat).value)
+    _check(lib().mega_equalize_slots(ctypes.byref(d), _stream(stream)))
+
+
+def demod_host(xhat, mod_order: int):
+    """Host demod through the library (numpy complex64 in, uint16 LLR bits
+    out, shape xhat.shape + (Q,))."""
+    import numpy as np
+    x = np.require(np.asarray(xhat, dtype=np.complex64), requirements="C")
+    out = np.zeros(x.shape + (mod_order,), dtype=np.uint16)
+    _check(lib().mega_demod_host(x.ctypes.data_as(ctypes.c_void_p), x.size,
+                                 mod_order, out.ctypes.data_as(ctypes.c_void_p)))
+    return out

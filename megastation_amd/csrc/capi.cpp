@@ -1,0 +1,237 @@
+// C ABI of libmegadetect: argument checks, error reporting, dispatch to the
+// HIP launchers. See include/megadetect.h for the contract of each call and
+// the reference interface it replaces.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/megadetect.h"
+#include "demod.h"
+#include "internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* why = nullptr) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (why) m += std::string(" (") + why + ")";
+  return fail(MEGA_ERR_HIP, m);
+}
+
+bool valid_q(uint64_t q) { return q == 2 || q == 4 || q == 6 || q == 8; }
+
+}  // namespace
+
+struct mega_beamform {
+  uint32_t num_bs, num_ue, batch_count;
+  int device;
+  int32_t* info;  // device, [batch_count]
+};
+
+extern "C" {
+
+const char* mega_version(void) { return "megadetect 0.1 (gfx950)"; }
+
+const char* mega_last_error(void) { return g_err.c_str(); }
+
+int mega_beamform_create(uint32_t num_bs, uint32_t num_ue, uint32_t batch_count,
+                         mega_beamform_t* out) {
+  if (!out) return fail(MEGA_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (num_bs == 0 || num_ue == 0 || batch_count == 0)
+    return fail(MEGA_ERR_INVALID_ARG, "num_bs, num_ue and batch_count must be > 0");
+  if (num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, "num_ue > 64 is not supported");
+  auto* bf = new mega_beamform{num_bs, num_ue, batch_count, 0, nullptr};
+  hipError_t e = hipGetDevice(&bf->device);
+  if (e == hipSuccess) e = hipMalloc(&bf->info, sizeof(int32_t) * batch_count);
+  if (e == hipSuccess) e = hipMemset(bf->info, 0, sizeof(int32_t) * batch_count);
+  if (e != hipSuccess) {
+    if (bf->info) (void)hipFree(bf->info);
+    delete bf;
+    return hip_fail(e, "mega_beamform_create");
+  }
+  *out = bf;
+  return MEGA_OK;
+}
+
+int mega_beamform_destroy(mega_beamform_t bf) {
+  if (!bf) return MEGA_OK;
+  hipError_t e = hipFree(bf->info);
+  delete bf;
+  if (e != hipSuccess) return hip_fail(e, "mega_beamform_destroy");
+  return MEGA_OK;
+}
+
+static int zf_checked(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
+                      const void* in, void* out, int32_t* info, int form,
+                      mega_stream_t stream, const char* who) {
+  if (form != MEGA_FORM_FP32)
+    return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": unknown form");
+  if (batch == 0) return MEGA_OK;
+  if (!in || !out) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
+  if (num_ue == 0 || num_bs == 0)
+    return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": zero size");
+  if (num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": num_ue > 64");
+  const char* why = nullptr;
+  hipError_t e = mega::detail::launch_zf(
+      num_ue, num_bs, batch, static_cast<const float2*>(in),
+      static_cast<float2*>(out), info, static_cast<hipStream_t>(stream), &why);
+  if (e != hipSuccess) return hip_fail(e, who, why);
+  return MEGA_OK;
+}
+
+int mega_beamform_uplink(mega_beamform_t bf, void* csi, uint32_t batch,
+                         mega_stream_t stream) {
+  if (!bf) return fail(MEGA_ERR_INVALID_ARG, "beamform handle is NULL");
+  if (batch > bf->batch_count)
+    return fail(MEGA_ERR_BATCH, "Batch size " + std::to_string(batch) +
+                                    " exceeds the maximum batch size " +
+                                    std::to_string(bf->batch_count));
+  return zf_checked(bf->num_bs, bf->num_ue, batch, csi, csi, bf->info,
+                    MEGA_FORM_FP32, stream, "mega_beamform_uplink");
+}
+
+int mega_beamform_downlink(const void* csi, void* precode, uint32_t num_ue,
+                           uint32_t num_bs, uint32_t batch,
+                           mega_stream_t stream) {
+  if (batch == 0) return MEGA_OK;
+  if (!csi || !precode) return fail(MEGA_ERR_INVALID_ARG, "NULL buffer");
+  if (num_ue == 0 || num_bs == 0) return fail(MEGA_ERR_INVALID_ARG, "zero size");
+  hipError_t e = mega::detail::launch_scale_absmax(
+      num_ue, num_bs, batch, static_cast<const float2*>(csi),
+      static_cast<float2*>(precode), static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "mega_beamform_downlink");
+  return MEGA_OK;
+}
+
+int mega_beamformer(mega_beamform_t bf, void* csi, void* precode,
+                    uint32_t batch, mega_stream_t stream) {
+  if (!bf) return fail(MEGA_ERR_INVALID_ARG, "beamform handle is NULL");
+  if (batch > bf->batch_count)
+    return fail(MEGA_ERR_BATCH, "Batch size exceeds the maximum batch size");
+  int r = mega_beamform_uplink(bf, csi, batch, stream);
+  if (r != MEGA_OK) return r;
+  return mega_beamform_downlink(csi, precode, bf->num_ue, bf->num_bs, batch,
+                                stream);
+}
+
+int mega_beamform_info(mega_beamform_t bf, int32_t* host_info, uint32_t batch,
+                       mega_stream_t stream) {
+  if (!bf || !host_info) return fail(MEGA_ERR_INVALID_ARG, "NULL argument");
+  if (batch > bf->batch_count) return fail(MEGA_ERR_BATCH, "batch > batch_count");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpyAsync(host_info, bf->info, sizeof(int32_t) * batch,
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "mega_beamform_info");
+  int failed = 0;
+  for (uint32_t i = 0; i < batch; ++i) failed += host_info[i] != 0;
+  return failed;
+}
+
+int mega_zf(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
+            const void* csi_in, void* w_out, int32_t* info_dev, int form,
+            mega_stream_t stream) {
+  return zf_checked(num_bs, num_ue, batch, csi_in, w_out, info_dev, form,
+                    stream, "mega_zf");
+}
+
+int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, "desc is NULL");
+  if (!valid_q(d->mod_order))
+    return fail(MEGA_ERR_UNSUPPORTED,
+                "unsupported modulation order " + std::to_string(d->mod_order));
+  if (d->form != MEGA_FORM_FP32) return fail(MEGA_ERR_UNSUPPORTED, "unknown form");
+  if (d->num_ue == 0 || d->num_bs == 0 || d->num_sc == 0 || d->num_syms == 0 ||
+      d->num_slots == 0)
+    return MEGA_OK;
+  if (!d->csi || !d->y || !d->llr) return fail(MEGA_ERR_INVALID_ARG, "NULL buffer");
+  if (d->sc_group == 0 || d->cb_size == 0)
+    return fail(MEGA_ERR_INVALID_ARG, "sc_group and cb_size must be > 0");
+  if (d->num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, "num_ue > 64");
+  if (d->num_slots > 65535 || d->num_syms > 65535)
+    return fail(MEGA_ERR_UNSUPPORTED, "more than 65535 slots or symbols per launch");
+  mega::detail::EqualizeArgs a;
+  a.csi = static_cast<const float2*>(d->csi);
+  a.y = static_cast<const float2*>(d->y);
+  a.llr = static_cast<_Float16*>(d->llr);
+  a.xhat = static_cast<float2*>(d->xhat);
+  a.U = d->num_ue;
+  a.B = d->num_bs;
+  a.N = d->num_sc;
+  a.sg = d->sc_group;
+  a.G = (d->num_sc + d->sc_group - 1) / d->sc_group;
+  a.S = d->num_syms;
+  a.slots = d->num_slots;
+  a.Q = d->mod_order;
+  a.ue_stride = d->ue_stride;
+  a.cb_size = d->cb_size;
+  a.cb_stride = d->cb_stride;
+  a.zeros = d->zeros;
+  a.valid_bits = d->valid_bits;
+  a.csi_slot = d->csi_slot_stride;
+  a.y_sym = d->y_sym_stride;
+  a.y_slot = d->y_slot_stride;
+  a.llr_sym = d->llr_sym_stride;
+  a.llr_slot = d->llr_slot_stride;
+  const char* why = nullptr;
+  hipError_t e = mega::detail::launch_equalize(a, static_cast<hipStream_t>(stream), &why);
+  if (e != hipSuccess) return hip_fail(e, "mega_equalize", why);
+  return MEGA_OK;
+}
+
+int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
+                  const void* csi, const void* bsvec, void* uevec,
+                  uint64_t ue_stride, uint64_t csi_skip, uint8_t mod_order,
+                  uint64_t cb_size, uint64_t cb_stride, uint64_t zeros,
+                  uint64_t valid_bits, mega_stream_t stream) {
+  if (row > 0xffffffffull || col > 0xffffffffull || batch_count > 0xffffffffull ||
+      csi_skip > 0xffffffffull)
+    return fail(MEGA_ERR_UNSUPPORTED, "dimension exceeds 32 bits");
+  mega_equalize_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.num_ue = (uint32_t)row;
+  d.num_bs = (uint32_t)col;
+  d.num_sc = (uint32_t)batch_count;
+  d.sc_group = (uint32_t)csi_skip;
+  d.num_syms = 1;
+  d.num_slots = 1;
+  d.mod_order = mod_order;
+  d.form = MEGA_FORM_FP32;
+  d.ue_stride = ue_stride;
+  d.cb_size = cb_size;
+  d.cb_stride = cb_stride;
+  d.zeros = zeros;
+  d.valid_bits = valid_bits;
+  d.csi = csi;
+  d.y = bsvec;
+  d.llr = uevec;
+  return mega_equalize_slots(&d, stream);
+}
+
+int mega_demod_host(const float* xhat_c64, uint64_t n, uint8_t mod_order,
+                    uint16_t* llr_f16) {
+  if (!valid_q(mod_order))
+    return fail(MEGA_ERR_UNSUPPORTED,
+                "unsupported modulation order " + std::to_string(mod_order));
+  if (n && (!xhat_c64 || !llr_f16)) return fail(MEGA_ERR_INVALID_ARG, "NULL buffer");
+  for (uint64_t i = 0; i < n; ++i) {
+    _Float16 v[8];
+    mega::demod::demod_any(xhat_c64[2 * i], xhat_c64[2 * i + 1], mod_order, v);
+    std::memcpy(llr_f16 + i * mod_order, v, sizeof(_Float16) * mod_order);
+  }
+  return MEGA_OK;
+}
+
+}  // extern "C"

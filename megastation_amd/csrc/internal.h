@@ -1,0 +1,39 @@
+// Launchers shared between the kernel translation units and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mega {
+namespace detail {
+
+// One equalize + demod launch over [slots][syms] OFDM symbols.
+struct EqualizeArgs {
+  const float2* csi;
+  const float2* y;
+  _Float16* llr;
+  float2* xhat;  // optional [slots][syms][U][N]
+  uint32_t U, B, N, sg, G, S, slots, Q;
+  uint64_t ue_stride, cb_size, cb_stride, zeros, valid_bits;
+  uint64_t csi_slot, y_sym, y_slot, llr_sym, llr_slot;
+};
+
+// Returns hipSuccess or the launch error; *why gets a static message for
+// argument combinations the kernels cannot take.
+hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t stream,
+                           const char** why);
+
+hipError_t launch_zf(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                     float2* out, int32_t* info, hipStream_t stream,
+                     const char** why);
+
+hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
+                               const float2* in, float2* out,
+                               hipStream_t stream);
+
+// Limits of the kernels (checked by the C ABI before launching).
+constexpr uint32_t kMaxUsers = 64;
+
+}  // namespace detail
+}  // namespace mega

@@ -1,0 +1,68 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, rejects bad arguments before touching the device, and its
+host demod is bit-identical to the oracle."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+import megastation_amd as m
+
+
+def test_library_exports_every_declared_symbol():
+    L = m.lib()
+    names = m.declared_symbols()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", m.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(names) <= exported
+
+
+def test_version_and_no_error_initially():
+    assert b"gfx950" in m.lib().mega_version()
+
+
+def test_unsupported_modulation_is_rejected_before_launch():
+    L = m.lib()
+    buf = (ctypes.c_uint16 * 64)()
+    rc = L.mega_equalize(4, 8, 16, ctypes.c_void_p(1), ctypes.c_void_p(1),
+                         ctypes.cast(buf, ctypes.c_void_p), 32, 16, 3, 32, 32, 0,
+                         32, None)
+    assert rc == m.MEGA_ERR_UNSUPPORTED
+    assert "modulation order 3" in m.last_error()
+    assert all(v == 0 for v in buf)
+
+
+def test_bad_beamform_arguments():
+    L = m.lib()
+    h = ctypes.c_void_p()
+    assert L.mega_beamform_create(0, 4, 2, ctypes.byref(h)) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_beamform_create(128, 65, 2, ctypes.byref(h)) == m.MEGA_ERR_UNSUPPORTED
+    assert L.mega_beamform_uplink(None, None, 1, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_zf(8, 4, 1, None, None, None, 0, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_zf(8, 4, 1, ctypes.c_void_p(1), ctypes.c_void_p(1), None, 7,
+                     None) == m.MEGA_ERR_UNSUPPORTED
+    assert L.mega_beamform_destroy(None) == m.MEGA_OK
+
+
+def test_equalize_desc_layout_matches_header():
+    # offsets of the C struct on LP64 (natural alignment)
+    D = m.EqualizeDesc
+    assert D.ue_stride.offset == 32
+    assert D.csi.offset == 72
+    assert D.xhat.offset == 136
+    assert ctypes.sizeof(D) == 144
+
+
+@pytest.mark.parametrize("Q", [2, 4, 6, 8])
+def test_host_demod_matches_oracle(orc, Q):
+    rng = np.random.default_rng(Q)
+    x = (rng.standard_normal(4000) * 1.3 + 1j * rng.standard_normal(4000) * 1.3)
+    x = np.concatenate([x, [0, 1e-9, 700 + 0j, -700j, 1e-3 - 1e-3j]]).astype(np.complex64)
+    assert np.array_equal(m.demod_host(x, Q), orc.demod(x, Q))
+    with pytest.raises(m.MegaError):
+        m.demod_host(x[:4], 5)

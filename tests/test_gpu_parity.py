@@ -1,0 +1,249 @@
+"""Parity of the HIP detection path (through the C ABI) with the CPU oracle.
+
+All tests run on cuda:0 of an MI355X. Inputs are seeded; sizes are chosen so
+the oracle finishes in seconds. Full-size workloads are checked through
+size-independent properties (EVM against FP64, demod bit-exactness of the
+kernel's own x_hat, run-to-run determinism)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import megastation_amd as m
+import oracle
+from megastation_amd import synth
+
+from gpu_helpers import (SENTINEL, check_symbol, dev_c64, gpu_equalize_slots,
+                         host_c64, host_u16, llr_buffer, make_slots,
+                         oracle_symbol_f64, torch)
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    torch().cuda.synchronize()
+
+
+# ----------------------------------------------------------------------------
+# reference fixtures through the drop-in entry points
+# ----------------------------------------------------------------------------
+
+
+def test_equalize_reference_fixture_bit_exact(golden):
+    """test/sgemv/test_sgemv.cc:126-159 via mega_equalize (raw overload)."""
+    print("case: sgemv fixture U=4 B=8 N=16 QPSK", flush=True)
+    U, B, N = 4, 8, 16
+    csi = dev_c64(golden["sgemv_csi"])
+    y = dev_c64(golden["sgemv_bsvec_kb"].reshape(N, B).T)
+    out = llr_buffer(U * 2 * N, 0)
+    m.Equalize.equalize_raw(U, B, N, csi, y, out, 2 * N, 16, 2, 2 * N, 2 * N, 0, 2 * N)
+    _sync()
+    assert np.array_equal(host_u16(out), golden["sgemv_equalize_f16bits"])
+
+
+def test_beamformer_reference_fixture(golden):
+    """test/beamform/test_beamform.cc:67-95: Beamform(8,4,2).beamformer."""
+    print("case: beamform fixture U=4 B=8 batch=2", flush=True)
+    t = torch()
+    a = np.concatenate([golden["beam_csi_in"]] * 2)
+    csi = dev_c64(a)
+    prec = t.zeros_like(csi)
+    bf = m.Beamform(8, 4, 2)
+    bf.beamformer(csi, prec, batch=2)
+    info = bf.info(2)
+    w, p = host_c64(csi), host_c64(prec)
+    assert not info.any()
+    for g in range(2):
+        assert np.abs(w[32 * g:32 * g + 32] - golden["beam_csi_out"]).max() < 1e-5
+        assert np.abs(p[32 * g:32 * g + 32] - golden["beam_precoder"]).max() < 1e-5
+    bf.close()
+
+
+# ----------------------------------------------------------------------------
+# ZF
+# ----------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("U,B,G,channel", [(4, 8, 3, "iid"), (16, 64, 75, "iid"),
+                                           (32, 256, 40, "kronecker"),
+                                           (8, 24, 17, "iid"), (64, 256, 6, "iid"),
+                                           (5, 300, 9, "iid")])
+def test_zf_matches_fp64_oracle(U, B, G, channel):
+    print(f"case: zf U={U} B={B} G={G} {channel}", flush=True)
+    rng = np.random.Generator(np.random.Philox(U * 1000 + B))
+    H = (synth.channel_iid(rng, G, B, U) if channel == "iid"
+         else synth.channel_kronecker(rng, G, B, U, 0.5, 0.3))
+    a = np.conj(H).astype(np.complex64)
+    _, w64, info64 = oracle.zf(a, U, B, G, "f64")
+    t = torch()
+    d_in = dev_c64(a)
+    d_out = t.zeros_like(d_in)
+    d_info = t.full((G,), -7, dtype=t.int32, device="cuda:0")
+    m.zf(d_in, d_out, B, U, G, info=d_info)
+    _sync()
+    w = host_c64(d_out).reshape(G, B * U).astype(np.complex128)
+    ref = w64.reshape(G, B * U)
+    rel = np.linalg.norm(w - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert rel.max() < 1e-4, rel.max()
+    assert np.array_equal(d_info.cpu().numpy(), info64)
+    assert np.array_equal(host_c64(d_in), a)       # out-of-place leaves input intact
+
+
+def test_zf_in_place_and_failed_groups():
+    print("case: zf in place with rank-deficient groups", flush=True)
+    rng = np.random.Generator(np.random.Philox(9))
+    U, B, G = 8, 32, 10
+    H = synth.channel_iid(rng, G, B, U)
+    H[3, :, 5] = H[3, :, 1]
+    H[7] = 0
+    a = np.conj(H).astype(np.complex64)
+    w32, _, info64 = oracle.zf(a, U, B, G, "f64")
+    bf = m.Beamform(B, U, G)
+    d = dev_c64(a)
+    bf.beamform_uplink(d)
+    info = bf.info(G)
+    assert info[7] == 1 and info[3] != 0
+    ok = info == 0
+    assert np.array_equal(ok, info64 == 0)
+    w = host_c64(d).reshape(G, -1)
+    assert not w[~ok].any()
+    ref = w32.reshape(G, -1)
+    assert np.abs(w[ok] - ref[ok]).max() < 1e-4 * np.abs(ref[ok]).max()
+    with pytest.raises(m.MegaError) as e:
+        bf.beamform_uplink(d, batch=G + 1)
+    assert e.value.code == m.MEGA_ERR_BATCH
+    bf.close()
+
+
+def test_beamform_downlink_normalisation(orc):
+    print("case: scale_absmax", flush=True)
+    rng = np.random.Generator(np.random.Philox(4))
+    U, B, G = 16, 64, 33
+    w = synth._cn(rng, (G, B, U))
+    t = torch()
+    d = dev_c64(w)
+    p = t.zeros_like(d)
+    m.Beamform.beamform_downlink(d, p, U, B, G)
+    _sync()
+    ref = orc.scale_absmax(w, U, B, G)
+    assert np.abs(host_c64(p).reshape(-1) - ref).max() < 1e-6
+
+
+# ----------------------------------------------------------------------------
+# equalize + demod
+# ----------------------------------------------------------------------------
+
+EQ_CASES = [
+    # U, B, N, sg, Q, layout
+    (4, 8, 16, 16, 2, "standalone"),
+    (16, 64, 1200, 16, 4, "standalone"),
+    (16, 64, 1200, 16, 6, "ldpc"),
+    (32, 256, 3276, 12, 6, "ldpc"),
+    (8, 64, 624, 12, 8, "standalone"),
+    (32, 128, 3276, 16, 8, "standalone"),   # partial last group (204.75 groups)
+    (64, 128, 600, 12, 4, "ldpc"),
+    (3, 5, 37, 7, 6, "standalone"),
+]
+
+
+@pytest.mark.parametrize("U,B,N,sg,Q,layout", EQ_CASES)
+def test_equalize_matches_oracle(U, B, N, sg, Q, layout):
+    print(f"case: equalize U={U} B={B} N={N} sg={sg} Q={Q} {layout}", flush=True)
+    S, slots = 2, 1
+    sl = make_slots(U * 7 + N, slots, U, B, N, sg, S, Q)
+    lay = synth.ldpc_uplink_layout(N, Q) if layout == "ldpc" else synth.standalone_layout(N, Q)
+    G = (N + sg - 1) // sg
+    w, _, _ = oracle.zf(sl[0].csi, U, B, G, "f64")   # equalize with exact weights
+    w = w.reshape(1, G, B, U)
+    y = np.stack([s.y for s in sl])
+    llr, xh = gpu_equalize_slots(w, y, U, B, N, sg, Q, lay, slots=slots, S=S)
+    for s in range(S):
+        llr64, x64 = oracle_symbol_f64(w[0], y[0, s], U, B, N, sg, Q, lay)
+        check_symbol(llr[0, s], xh[0, s], llr64, x64, N, sg, Q, lay)
+
+
+def test_equalize_slots_equals_per_symbol_calls():
+    """Batched launch == one mega_equalize per symbol, bit for bit."""
+    print("case: slots vs per-symbol", flush=True)
+    U, B, N, sg, Q, S, slots = 16, 64, 240, 16, 4, 3, 2
+    sl = make_slots(77, slots, U, B, N, sg, S, Q)
+    lay = synth.standalone_layout(N, Q)
+    csi = np.stack([s.csi for s in sl])
+    y = np.stack([s.y for s in sl])
+    llr, _ = gpu_equalize_slots(csi, y, U, B, N, sg, Q, lay, slots=slots, S=S,
+                                want_xhat=False)
+    for s in range(slots):
+        for t_ in range(S):
+            out = llr_buffer(U * lay.ue_stride)
+            m.Equalize.equalize_raw(U, B, N, dev_c64(csi[s]), dev_c64(y[s, t_]), out,
+                                    lay.ue_stride, sg, Q, lay.cb_size, lay.cb_stride,
+                                    lay.zeros, lay.valid_bits)
+            _sync()
+            assert np.array_equal(host_u16(out).reshape(U, -1), llr[s, t_])
+
+
+def test_equalize_ownership_and_determinism():
+    """Two runs with different sentinels: the same positions are written, with
+    the same values (each (user, subcarrier, bit) stored exactly once, no
+    stores into a neighbouring group's outputs at sg = 12)."""
+    print("case: ownership sg=12", flush=True)
+    U, B, N, sg, Q, S = 32, 64, 1200, 12, 6, 3
+    sl = make_slots(5, 1, U, B, N, sg, S, Q)
+    lay = synth.ldpc_uplink_layout(N, Q)
+    csi = sl[0].csi[None]
+    y = sl[0].y[None]
+    a, _ = gpu_equalize_slots(csi, y, U, B, N, sg, Q, lay, slots=1, S=S,
+                              want_xhat=False, fill=SENTINEL)
+    b, _ = gpu_equalize_slots(csi, y, U, B, N, sg, Q, lay, slots=1, S=S,
+                              want_xhat=False, fill=0x7E02)
+    wa, wb = a != SENTINEL, b != 0x7E02
+    assert np.array_equal(wa, wb)
+    assert np.array_equal(a[wa], b[wb])
+    per_user = wa.sum(-1)
+    assert np.all(per_user == min(N * Q, lay.valid_bits))
+
+
+def test_equalize_rejects_bad_modulation_without_writing():
+    print("case: bad Q", flush=True)
+    out = llr_buffer(64, 0)
+    with pytest.raises(m.MegaError) as e:
+        m.Equalize.equalize_raw(4, 8, 16, dev_c64(np.ones(32)), dev_c64(np.ones(128)),
+                                out, 32, 16, 5, 32, 32, 0, 32)
+    assert e.value.code == m.MEGA_ERR_UNSUPPORTED
+    _sync()
+    assert not host_u16(out).any()
+
+
+# ----------------------------------------------------------------------------
+# end to end at the cell shapes of the benchmark configurations
+# ----------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("U,B,N,sg,Q,channel,snr", [
+    (16, 64, 1200, 16, 4, "iid", 20.0),
+    (16, 64, 1200, 16, 6, "iid", 30.0),
+    (32, 256, 3276, 12, 6, "kronecker", 25.0),
+])
+def test_detect_end_to_end_evm(U, B, N, sg, Q, channel, snr):
+    """ZF on the GPU, then equalize+demod: EVM within 0.1 dB of the FP64
+    oracle's, symbol-level checks as in test_equalize_matches_oracle."""
+    print(f"case: e2e U={U} B={B} N={N} sg={sg} Q={Q} {channel}", flush=True)
+    S = 2
+    sl = make_slots(U + N + Q, 1, U, B, N, sg, S, Q, snr, channel)[0]
+    lay = synth.ldpc_uplink_layout(N, Q)
+    G = (N + sg - 1) // sg
+    t = torch()
+    d_a = dev_c64(sl.csi)
+    d_w = t.zeros_like(d_a)
+    m.zf(d_a, d_w, B, U, G)
+    _sync()
+    w_gpu = host_c64(d_w).reshape(1, G, B, U)
+    llr, xh = gpu_equalize_slots(w_gpu, sl.y[None], U, B, N, sg, Q, lay, slots=1, S=S)
+    w64, _, _ = oracle.zf(sl.csi, U, B, G, "f64")
+    for s in range(S):
+        llr64, x64 = oracle_symbol_f64(w64, sl.y[s], U, B, N, sg, Q, lay)
+        check_symbol(llr[0, s], xh[0, s], llr64, x64, N, sg, Q, lay)
+        e_gpu = synth.evm_db(xh[0, s], sl.x[s])
+        e_64 = synth.evm_db(x64, sl.x[s])
+        assert abs(e_gpu - e_64) < 0.1, (e_gpu, e_64)
