@@ -237,14 +237,18 @@ static int zf_group_f64(uint32_t U, uint32_t B, const float *a, double *w,
       L[2 * (i * U + j)] = re;
       L[2 * (i * U + j) + 1] = im;
     }
-  /* Cholesky, lower, left-looking */
+  /* Cholesky, lower, left-looking. A pivot (Schur-complement diagonal) not
+   * above U * 2^-23 * G_kk marks the group numerically singular, the
+   * product's FP32 criterion. */
+  const double tol = (double)U * 0x1p-23;
   for (uint32_t k = 0; k < U; ++k) {
-    double d = L[2 * (k * U + k)];
+    const double gkk = L[2 * (k * U + k)];
+    double d = gkk;
     for (uint32_t j = 0; j < k; ++j) {
       double r = L[2 * (k * U + j)], m = L[2 * (k * U + j) + 1];
       d -= r * r + m * m;
     }
-    if (!(d > 0.) || !isfinite(d)) return (int)k + 1;
+    if (!(d > tol * gkk) || !isfinite(d)) return (int)k + 1;
     d = sqrt(d);
     L[2 * (k * U + k)] = d;
     L[2 * (k * U + k) + 1] = 0.;
@@ -330,13 +334,15 @@ static int zf_group_f32(uint32_t U, uint32_t B, const float *a, float *w,
       L[2 * (i * U + j)] = re;
       L[2 * (i * U + j) + 1] = im;
     }
+  const float tol = (float)U * 0x1p-23f;
   for (uint32_t k = 0; k < U; ++k) {
-    float d = L[2 * (k * U + k)];
+    const float gkk = L[2 * (k * U + k)];
+    float d = gkk;
     for (uint32_t j = 0; j < k; ++j) {
       float r = L[2 * (k * U + j)], m = L[2 * (k * U + j) + 1];
       d -= r * r + m * m;
     }
-    if (!(d > 0.f) || !isfinite(d)) return (int)k + 1;
+    if (!(d > tol * gkk) || !isfinite(d)) return (int)k + 1;
     d = sqrtf(d);
     L[2 * (k * U + k)] = d;
     L[2 * (k * U + k) + 1] = 0.f;

@@ -61,8 +61,9 @@ int orc_equalize_f64(uint64_t U, uint64_t B, uint64_t N, const float *csi,
 /* ---- zero-forcing weights (src/operators/beamform/beamform.cu:74-122) ----
  * For each group: Gram = A A^H, lower Cholesky, W = Gram^-1 A by forward and
  * backward substitution. in and out may alias (the reference works in place).
- * info[g] = 0 on success, else k+1 for the first non-positive pivot; the
- * group's W is then all zeros. Returns the number of failed groups. */
+ * info[g] = 0 on success, else k+1 for the first pivot (squared Cholesky
+ * diagonal) not above U * 2^-23 * Gram[k][k] (numerically singular in FP32);
+ * the group's W is then all zeros. Returns the number of failed groups. */
 int orc_zf_f64(uint32_t U, uint32_t B, uint32_t batch, const float *in,
                float *out, double *out64, int32_t *info);
 int orc_zf_f32(uint32_t U, uint32_t B, uint32_t batch, const float *in,
