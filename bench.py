@@ -5,9 +5,11 @@ one MI355X.
 Workload (default): a 64-antenna x 16-user cell, 1200 data subcarriers in
 groups of 16 (75 groups), 13 data OFDM symbols per slot, 16-QAM at 20 dB SNR
 over i.i.d. CN(0,1) channels, batched over 200 slots. One step = ZF weights of
-all 200 x 75 groups (one launch) + equalize/demod of all 200 x 13 symbols (one
-launch), inputs resident in HBM. Synthetic data generated on the GPU with a
-seeded Philox generator.
+all 200 x 75 groups + equalize/demod of all 200 x 13 symbols, inputs resident
+in HBM. Two forms are timed: the fused kernel (ZF and equalize in one launch,
+W kept on chip; the headline value) and the drop-in plain form (ZF kernel
+writing W, then the equalize kernel). Synthetic data generated on the GPU with
+a seeded Philox generator.
 
 Prints one JSON line: value = detected symbols/s = U*N*S*slots / step time;
 roofline of the dominant kernel (equalize/demod, HBM-bound) from HIP events
@@ -86,6 +88,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of a known size (1 GiB), whose "
+                         "FETCH_SIZE / WRITE_SIZE counters scale the equalize traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -115,7 +120,8 @@ def main():
                  y_sym_stride=B * N, y_slot_stride=S * B * N,
                  llr_sym_stride=U * lay.ue_stride, llr_slot_stride=S * U * lay.ue_stride)
 
-    def step(evs=None):
+    def step_plain(evs=None):
+        """Drop-in form: ZF kernel (W to HBM) then the equalize kernel."""
         if evs:
             evs[0].record(stream)
         m.zf(csi, w, B, U, slots * G, stream=stream)
@@ -125,42 +131,78 @@ def main():
         if evs:
             evs[2].record(stream)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    def step_fused(evs=None):
+        """Fused form: ZF + equalize + demod in one kernel, W kept on chip."""
+        if evs:
+            evs[0].record(stream)
+        m.detect_slots(csi, y, llr, stream=stream, **eq_kw)
+        if evs:
+            evs[1].record(stream)
+            evs[2].record(stream)
 
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0.record(stream)
-    for i in range(args.steps):
-        step(evs[i])
-    t1.record(stream)
-    torch.cuda.synchronize()
-    total_ms = t0.elapsed_time(t1)
-    ms_step = total_ms / args.steps
-    zf_ms = [e[0].elapsed_time(e[1]) for e in evs]
-    eq_ms = [e[1].elapsed_time(e[2]) for e in evs]
-    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
+               for _ in range(args.steps)]
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record(stream)
+        for i in range(args.steps):
+            step(evs[i])
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / args.steps
+        k1 = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
+        k2 = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
+        return ms, k1, k2
+
+    forms = {}
+    forms["fused"] = timed(step_fused) if m.detect_fused_shape(U, B) else None
+    print(f"[bench] fused form timed ({args.steps} steps)", file=sys.stderr, flush=True)
+    forms["plain"] = timed(step_plain)
+    print(f"[bench] plain form timed ({args.steps} steps)", file=sys.stderr, flush=True)
 
     symbols = U * N * S * slots
-    value = symbols / (ms_step * 1e-3)
-    eq_bytes = 8 * B * N * S * slots + 8 * U * B * G * slots + 2 * U * Q * N * S * slots
-    eq_avg = statistics.mean(eq_ms)
-    achieved = eq_bytes / (eq_avg * 1e-3) / 1e9
+    io_bytes = 8 * B * N * S * slots + 8 * U * B * G * slots + 2 * U * Q * N * S * slots
     zf_flops = G * slots * (4 * U * (U + 1) * B + 8 * U * U * B + (4 / 3) * U ** 3)
+    eq_flops = 8 * U * B * N * S * slots
+    plain_ms, zf_ms, eq_ms = forms["plain"]
+    # Headline = the faster of the measured forms (both are FP32, same results).
+    if forms["fused"] is not None and forms["fused"][0] < plain_ms:
+        head_form, ms_step = "fused", forms["fused"][0]
+        dom_kernel, dom_ms = "detect_fused (ZF + equalize + demod)", forms["fused"][1]
+    else:
+        head_form, ms_step = "plain", plain_ms
+        dom_kernel, dom_ms = "equalize_demod", eq_ms
+    value = symbols / (ms_step * 1e-3)
+    achieved = io_bytes / (dom_ms * 1e-3) / 1e9
 
-    # ---- EVM of the GPU path on slot 0 (x_hat side output, untimed) --------
+    # ---- EVM of the headline form on slot 0 (x_hat side output, untimed) ----
     xh = torch.zeros(S * U * N, dtype=torch.complex64, device=dev)
     kw1 = dict(eq_kw, num_slots=1)
     llr1 = torch.empty(S * U * lay.ue_stride, dtype=torch.int16, device=dev)
-    m.equalize_slots(w[0], y[0], llr1, xhat=xh, stream=stream, **kw1)
+    if head_form == "fused":
+        m.detect_slots(csi[0], y[0], llr1, xhat=xh, stream=stream, **kw1)
+    else:
+        m.zf(csi[0], w[0], B, U, G, stream=stream)
+        m.equalize_slots(w[0], y[0], llr1, xhat=xh, stream=stream, **kw1)
     torch.cuda.synchronize()
     xh_h = xh.cpu().numpy().reshape(S, U, N)
     x0 = x[0].cpu().numpy()
     evm_gpu = synth.evm_db(xh_h, x0)
+    print(f"[bench] {head_form}: {ms_step:.3f} ms/step, plain: {plain_ms:.3f} ms/step",
+          file=sys.stderr, flush=True)
 
     result = {
         "metric": METRIC, "value": value, "unit": "symbols/s", "n_gpus": 1,
@@ -170,19 +212,26 @@ def main():
         "config": {"workload": args.workload, "antennas": B, "users": U,
                    "subcarriers": N, "sc_group": sg, "groups": G, "data_symbols": S,
                    "slots": slots, "qam": 1 << Q, "snr_db": cfg["snr"],
-                   "channel": cfg["channel"], "form": "fp32",
+                   "channel": cfg["channel"], "form": head_form,
                    "parallelism": "1 GPU, groups x slots"},
-        "roofline": {"bound": "hbm", "kernel": "equalize_demod",
+        "roofline": {"bound": "hbm", "kernel": dom_kernel,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes": eq_bytes, "avg_ms": eq_avg},
-        "kernels": {"zf_ms": statistics.mean(zf_ms), "equalize_ms": eq_avg,
-                    "zf_tflops": zf_flops / (statistics.mean(zf_ms) * 1e-3) / 1e12},
-        "evm": {"gpu_db": evm_gpu},
+                     "algorithmic_bytes": io_bytes, "avg_ms": dom_ms},
+        "forms": {
+            "fused": ({"symbols_per_s": symbols / (forms["fused"][0] * 1e-3),
+                       "ms_per_step": forms["fused"][0]} if forms["fused"] else None),
+            "plain": {"symbols_per_s": symbols / (plain_ms * 1e-3),
+                      "ms_per_step": plain_ms, "zf_ms": zf_ms, "equalize_ms": eq_ms,
+                      "zf_tflops": zf_flops / (zf_ms * 1e-3) / 1e12,
+                      "equalize_gbs": (io_bytes) / (eq_ms * 1e-3) / 1e9,
+                      "equalize_tflops": eq_flops / (eq_ms * 1e-3) / 1e12},
+        },
+        "evm": {"gpu_db": evm_gpu, "form": head_form},
     }
 
     if not args.no_cpu_baseline:
-        result["cpu_baseline"], evm64 = cpu_baseline(cfg, lay, csi, y, x, xh_h,
+        result["cpu_baseline"], evm64 = cpu_baseline(cfg, lay, csi, y, x,
                                                      args.cpu_seconds)
         result["evm"]["fp64_db"] = evm64
         result["evm"]["delta_db"] = evm_gpu - evm64
@@ -190,7 +239,7 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(cfg, lay, csi, y, x, xh_gpu, budget_s):
+def cpu_baseline(cfg, lay, csi, y, x, budget_s):
     """The oracle's OpenMP restatement (FP32 Gram -> Cholesky -> W, then
     equalize + demod per group) timed on the host cores over a bounded sample
     of the same slots; plus the FP64 oracle's EVM on slot 0."""

@@ -137,6 +137,24 @@ typedef struct {
 
 int mega_equalize_slots(const mega_equalize_desc *desc, mega_stream_t stream);
 
+/* ---- Fused detection ------------------------------------------------------
+ * ZF weights and equalize + demod of every symbol of every slot in one call:
+ * eq.csi holds the channel estimates A_g = H_g^H (not W). For U <= 16 and
+ * B <= 256 one kernel does both per (slot, group) and W never touches HBM
+ * unless w_out is given; other shapes run the ZF and equalize kernels back to
+ * back through w_out, which is then required. Results equal
+ * mega_zf(eq.csi -> w_out) followed by mega_equalize_slots(w_out, ...).
+ * No single reference counterpart: it fuses Beamform::beamform_uplink
+ * (beamform.cu:144-152) with the S Equalize::equalize calls of a slot
+ * (process.cc:172-185). */
+typedef struct {
+  mega_equalize_desc eq;
+  void *w_out;       /* optional: W, same layout and strides as eq.csi */
+  int32_t *info_dev; /* optional: device int32[num_slots * G] group status */
+} mega_detect_desc;
+
+int mega_detect_slots(const mega_detect_desc *desc, mega_stream_t stream);
+
 /* ---- host demodulation ----------------------------------------------------
  * Replaces the host side of demodQPSK / demod16QAM / demod64QAM /
  * demod256QAM (demodulation.h:30-67, __device__ __host__): n complex64

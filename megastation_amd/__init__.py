@@ -58,6 +58,12 @@ class EqualizeDesc(ctypes.Structure):
     ]
 
 
+class DetectDesc(ctypes.Structure):
+    """Mirror of ``mega_detect_desc`` (include/megadetect.h)."""
+    _fields_ = [("eq", EqualizeDesc), ("w_out", ctypes.c_void_p),
+                ("info_dev", ctypes.c_void_p)]
+
+
 def declared_symbols() -> list[str]:
     """Names of the functions include/megadetect.h declares."""
     text = open(HEADER_PATH).read()
@@ -89,6 +95,7 @@ def lib() -> ctypes.CDLL:
         "mega_equalize": ([u64, u64, u64, P, P, P, u64, u64, u8, u64, u64, u64,
                            u64, P], i32),
         "mega_equalize_slots": ([ctypes.POINTER(EqualizeDesc), P], i32),
+        "mega_detect_slots": ([ctypes.POINTER(DetectDesc), P], i32),
         "mega_demod_host": ([P, u64, u8, P], i32),
     }
     for name, (args, res) in sig.items():
@@ -225,6 +232,27 @@ def equalize_slots(csi, y, llr, *, num_ue, num_bs, num_sc, sc_group, num_syms,
                      y_sym_stride, y_slot_stride, _ptr(llr).value, llr_sym_stride,
                      llr_slot_stride, _ptr(xhat).value)
     _check(lib().mega_equalize_slots(ctypes.byref(d), _stream(stream)))
+
+
+def detect_fused_shape(num_ue: int, num_bs: int) -> bool:
+    """Whether mega_detect_slots runs one fused kernel for this cell shape
+    (otherwise it runs ZF then equalize through w_out)."""
+    return num_ue <= 16 and num_bs <= 256
+
+
+def detect_slots(csi, y, llr, *, num_ue, num_bs, num_sc, sc_group, num_syms,
+                 num_slots, mod_order, ue_stride, cb_size, cb_stride, zeros,
+                 valid_bits, csi_slot_stride, y_sym_stride, y_slot_stride,
+                 llr_sym_stride, llr_slot_stride, xhat=None, w_out=None,
+                 info=None, form: int = FORM_FP32, stream=None):
+    """Fused ZF + equalize + demod: csi holds the channel estimates A = H^H."""
+    e = EqualizeDesc(num_ue, num_bs, num_sc, sc_group, num_syms, num_slots,
+                     mod_order, form, ue_stride, cb_size, cb_stride, zeros,
+                     valid_bits, _ptr(csi).value, csi_slot_stride, _ptr(y).value,
+                     y_sym_stride, y_slot_stride, _ptr(llr).value, llr_sym_stride,
+                     llr_slot_stride, _ptr(xhat).value)
+    d = DetectDesc(e, _ptr(w_out).value, _ptr(info).value)
+    _check(lib().mega_detect_slots(ctypes.byref(d), _stream(stream)))
 
 
 def demod_host(xhat, mod_order: int):

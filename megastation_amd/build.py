@@ -20,7 +20,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmegadetect.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["capi.cpp", "equalize.hip", "zf.hip"]
+SOURCES = ["capi.cpp", "equalize.hip", "zf.hip", "detect.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -146,23 +146,27 @@ int mega_zf(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
                     stream, "mega_zf");
 }
 
-int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
-  if (!d) return fail(MEGA_ERR_INVALID_ARG, "desc is NULL");
+static int eq_args_checked(const mega_equalize_desc* d, mega::detail::EqualizeArgs& a,
+                           bool& empty, const char* who) {
+  empty = false;
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": desc is NULL");
   if (!valid_q(d->mod_order))
     return fail(MEGA_ERR_UNSUPPORTED,
                 "unsupported modulation order " + std::to_string(d->mod_order));
   if (d->form != MEGA_FORM_FP32) return fail(MEGA_ERR_UNSUPPORTED, "unknown form");
   if (d->num_ue == 0 || d->num_bs == 0 || d->num_sc == 0 || d->num_syms == 0 ||
-      d->num_slots == 0)
+      d->num_slots == 0) {
+    empty = true;
     return MEGA_OK;
-  if (!d->csi || !d->y || !d->llr) return fail(MEGA_ERR_INVALID_ARG, "NULL buffer");
+  }
+  if (!d->csi || !d->y || !d->llr)
+    return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
   if (d->sc_group == 0 || d->cb_size == 0)
     return fail(MEGA_ERR_INVALID_ARG, "sc_group and cb_size must be > 0");
   if (d->num_ue > mega::detail::kMaxUsers)
     return fail(MEGA_ERR_UNSUPPORTED, "num_ue > 64");
   if (d->num_slots > 65535 || d->num_syms > 65535)
     return fail(MEGA_ERR_UNSUPPORTED, "more than 65535 slots or symbols per launch");
-  mega::detail::EqualizeArgs a;
   a.csi = static_cast<const float2*>(d->csi);
   a.y = static_cast<const float2*>(d->y);
   a.llr = static_cast<_Float16*>(d->llr);
@@ -185,9 +189,34 @@ int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
   a.y_slot = d->y_slot_stride;
   a.llr_sym = d->llr_sym_stride;
   a.llr_slot = d->llr_slot_stride;
+  return MEGA_OK;
+}
+
+int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
+  mega::detail::EqualizeArgs a;
+  bool empty;
+  int r = eq_args_checked(d, a, empty, "mega_equalize_slots");
+  if (r != MEGA_OK || empty) return r;
   const char* why = nullptr;
   hipError_t e = mega::detail::launch_equalize(a, static_cast<hipStream_t>(stream), &why);
   if (e != hipSuccess) return hip_fail(e, "mega_equalize", why);
+  return MEGA_OK;
+}
+
+int mega_detect_slots(const mega_detect_desc* d, mega_stream_t stream) {
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, "mega_detect_slots: desc is NULL");
+  mega::detail::EqualizeArgs a;
+  bool empty;
+  int r = eq_args_checked(&d->eq, a, empty, "mega_detect_slots");
+  if (r != MEGA_OK || empty) return r;
+  if (!d->w_out && !mega::detail::detect_fused_supported(a.U, a.B))
+    return fail(MEGA_ERR_INVALID_ARG,
+                "mega_detect_slots: U > 16 or B > 256 needs w_out");
+  const char* why = nullptr;
+  hipError_t e = mega::detail::launch_detect(a, static_cast<float2*>(d->w_out),
+                                             d->info_dev,
+                                             static_cast<hipStream_t>(stream), &why);
+  if (e != hipSuccess) return hip_fail(e, "mega_detect_slots", why);
   return MEGA_OK;
 }
 

@@ -1,0 +1,129 @@
+// Fused detection: ZF weights and equalize + demod of every symbol of a slot
+// in one workgroup per (slot, subcarrier group).
+//
+// The drop-in path runs beamform_uplink (W written over the CSI buffer,
+// beamform.cu:74-122) and then one equalize per symbol
+// (equalize.cu:20-51), so W makes a round trip through HBM between the two.
+// Here the workgroup computes W_g = (A_g A_g^H)^-1 A_g into LDS (zf_core.h),
+// loads its equalize operands from there and streams the slot's y through the
+// MFMA GEMM (eq_core.h): HBM traffic is A + y + LLR only. W is also written
+// to w_out when the caller wants it (downlink precoding, or to reuse it for
+// later symbols of the frame). Results equal ZF followed by equalize of the
+// same kernels' arithmetic.
+//
+// Shapes: up to 16 users (one 16-row Gram tile) and 256 antennas (four waves
+// of 64 antennas); other shapes run the two kernels back to back through
+// w_out.
+#include "zf_core.h"
+
+namespace mega {
+namespace detail {
+namespace {
+
+constexpr uint32_t kLdw = 24;  // LDS row of W (complex): conflict-free operand reads
+
+template <int MT, int KS, int NW, int Q>
+__global__ __launch_bounds__(64 * NW) void detect_fused(const EqParams p) {
+  __shared__ ZfShared<1> sh;
+  extern __shared__ float sDyn[];
+  float2* sW = reinterpret_cast<float2*>(sDyn);
+  float* sRed = sDyn + 2 * kLdw * p.B;
+  const uint32_t wk = threadIdx.x >> 6;
+
+  const uint32_t logical = xcd_linear(blockIdx.x, gridDim.x);
+  const uint32_t g = logical % p.G;
+  const uint32_t slot = logical / p.G;
+  const uint64_t goff = slot * p.csi_slot + (uint64_t)g * p.U * p.B;
+  float2* wout = p.w_out ? p.w_out + goff : nullptr;
+
+  const int fail = zf_group<1, 64 * NW>(p.csi + goff, p.U, p.B, sh, sW, kLdw, wout);
+  if (threadIdx.x == 0 && p.info) p.info[(uint64_t)slot * p.G + g] = fail;
+  __syncthreads();
+
+  float wa1[MT][KS], wa2[MT][KS];
+  load_w_operands<MT, KS>(sW, kLdw, p.U, p.B, 0, wk, wa1, wa2);
+  const uint32_t nk = min(p.sg, p.N - g * p.sg);
+  const uint32_t ntiles = (nk * p.S + 15) / 16;
+  eq_tiles<MT, KS, Q>(p, wa1, wa2, slot, g, 0, wk, 0, ntiles, sRed);
+}
+
+template <int MT, int NW, int Q>
+hipError_t launch_one(const EqParams& p, dim3 grid, size_t lds, hipStream_t st) {
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&detect_fused<MT, 16, NW, Q>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((detect_fused<MT, 16, NW, Q>), grid, dim3(64 * NW), lds, st, p);
+  return hipGetLastError();
+}
+
+template <int MT, int NW>
+hipError_t launch_q(const EqParams& p, dim3 grid, size_t lds, uint32_t Q,
+                    hipStream_t st) {
+  switch (Q) {
+    case 2: return launch_one<MT, NW, 2>(p, grid, lds, st);
+    case 4: return launch_one<MT, NW, 4>(p, grid, lds, st);
+    case 6: return launch_one<MT, NW, 6>(p, grid, lds, st);
+    case 8: return launch_one<MT, NW, 8>(p, grid, lds, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+bool detect_fused_supported(uint32_t U, uint32_t B) { return U <= 16 && B <= 256; }
+
+hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
+                         hipStream_t st, const char** why) {
+  if (a.U == 0 || a.B == 0 || a.N == 0 || a.sg == 0 || a.S == 0 || a.slots == 0)
+    return hipSuccess;
+  if (!detect_fused_supported(a.U, a.B)) {
+    if (!w_out) {
+      *why = "shape outside the fused kernel (U > 16 or B > 256) needs w_out";
+      return hipErrorInvalidValue;
+    }
+    hipError_t e = launch_zf(a.U, a.B, a.slots * a.G, a.csi, w_out, info, st, why);
+    if (e != hipSuccess) return e;
+    EqualizeArgs b = a;
+    b.csi = w_out;
+    return launch_equalize(b, st, why);
+  }
+  EqParams p;
+  hipError_t e = fill_eq_params(a, p, why);
+  if (e != hipSuccess) return e;
+  p.w_out = w_out;
+  p.info = info;
+  const uint32_t MT = a.U <= 8 ? 1 : 2;
+  // KS = 16 antenna steps (64 antennas) per wave; 3 waves' worth run as 4
+  // (the last wave's antennas are all beyond B and contribute zeros).
+  const uint32_t need = (a.B + 63) / 64;
+  const uint32_t nw = need == 1 ? 1 : need == 2 ? 2 : 4;
+  p.nmg = 1;
+  p.nwk = nw;
+  p.tiles_per_wg = p.col_tiles;
+  p.chunks = 1;
+  const uint64_t nblocks = (uint64_t)a.slots * a.G;
+  if (nblocks > 0x7fffffffull) {
+    *why = "grid too large";
+    return hipErrorInvalidValue;
+  }
+  const size_t lds = (size_t)kLdw * a.B * sizeof(float2) +
+                     (nw > 1 ? (size_t)2 * nw * MT * 256 * sizeof(float) : 0);
+  dim3 grid((uint32_t)nblocks);
+  switch (MT * 10 + nw) {
+    case 11: return launch_q<1, 1>(p, grid, lds, a.Q, st);
+    case 12: return launch_q<1, 2>(p, grid, lds, a.Q, st);
+    case 14: return launch_q<1, 4>(p, grid, lds, a.Q, st);
+    case 21: return launch_q<2, 1>(p, grid, lds, a.Q, st);
+    case 22: return launch_q<2, 2>(p, grid, lds, a.Q, st);
+    case 24: return launch_q<2, 4>(p, grid, lds, a.Q, st);
+    default:
+      *why = "no fused configuration";
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace detail
+}  // namespace mega

@@ -1,0 +1,273 @@
+// Device building blocks of the fused equalize + demod GEMM, shared by the
+// stand-alone equalize kernel and the fused ZF + equalize kernel.
+//
+// Per subcarrier group the work of a slot is one complex GEMM,
+//   X (U x sg*S) = W_g (U x B) * Y (B x sg*S),
+// on the FP32 matrix cores (v_mfma_f32_16x16x4_f32: FP32 products, FP32
+// accumulation). The complex product is split into two real MFMAs per step of
+// 4 antennas with the output rows interleaved as (u re, u im):
+//   MFMA1: A1[2u][k] = Re W[u,b], A1[2u+1][k] = Im W[u,b], B1[k][n] = Re y[b,n]
+//   MFMA2: A2[2u][k] = -Im W[u,b], A2[2u+1][k] = Re W[u,b], B2[k][n] = Im y[b,n]
+// so D[2u][n] = Re x, D[2u+1][n] = Im x: every lane ends up holding whole
+// equalized symbols (users 2q, 2q+1 of its 16-row tile at column n) for the
+// fused demod. A 16-row M tile covers 8 users; a 16-column N tile covers 16
+// (symbol, subcarrier) columns of the group.
+//
+// Each wave keeps its slice of W_g (MT M tiles x KS antenna steps) in
+// registers and streams y straight from HBM (lane = column: 16 lanes read 128
+// contiguous bytes of an antenna row; one float2 feeds both MFMAs), with the
+// next tile's y in flight during the current tile's MFMAs. When B is wider
+// than one wave's KS steps, the waves of a workgroup split the antennas and
+// their partial sums are reduced through LDS (double-buffered, one barrier per
+// tile).
+#pragma once
+
+#include "demod.h"
+#include "internal.h"
+
+namespace mega {
+namespace detail {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+struct EqParams {
+  const float2* csi;  // W (equalize) or A = H^H (fused detect), [slots][G][B][U]
+  const float2* y;
+  _Float16* llr;
+  float2* xhat;
+  float2* w_out;      // fused detect: optional W output, same layout as csi
+  int32_t* info;      // fused detect: optional per-group status [slots*G]
+  uint32_t U, B, N, sg, S, G, slots;
+  uint32_t nmg;              // workgroups per tile: groups of 8*MT users
+  uint32_t nwk;              // waves per workgroup: antenna splits
+  uint32_t col_tiles;        // 16-column tiles of a full group's sg*S columns
+  uint32_t tiles_per_wg, chunks;
+  uint64_t ue_stride, cb_size, cb_stride, zeros, valid_bits;
+  uint64_t csi_slot, y_sym, y_slot, llr_sym, llr_slot;
+  uint32_t cb_size32;  // cb_size (the host checks N*Q and cb_size fit 32 bits)
+  uint32_t store32;    // every LLR offset even and the base 4-byte aligned
+};
+
+// Where the Q LLRs of subcarrier k go within a user row
+// (equalize_kernel.cuh:461-478), and whether they can be written as Q/2
+// aligned 32-bit words.
+struct LlrSlot {
+  uint64_t off;
+  uint32_t bit;
+  bool full;
+};
+
+template <int Q>
+__device__ __forceinline__ LlrSlot llr_slot(const EqParams& p, uint32_t k) {
+  LlrSlot s;
+  s.bit = k * Q;
+  const uint32_t cb = s.bit / p.cb_size32;
+  s.off = (uint64_t)cb * p.cb_stride + p.zeros + (s.bit - cb * p.cb_size32);
+  s.full = p.store32 && s.bit + Q <= p.valid_bits && s.off + Q <= p.ue_stride;
+  return s;
+}
+
+template <int Q>
+__device__ __forceinline__ void store_llrs(const EqParams& p, _Float16* row,
+                                           const LlrSlot& s, float re, float im) {
+  _Float16 v[Q];
+  demod::demod<Q>(re, im, v);
+  _Float16* dst = row + s.off;
+  if (s.full) {
+#pragma unroll
+    for (int j = 0; j < Q / 2; ++j) {
+      uint32_t lo = __builtin_bit_cast(uint16_t, v[2 * j]);
+      uint32_t hi = __builtin_bit_cast(uint16_t, v[2 * j + 1]);
+      reinterpret_cast<uint32_t*>(dst)[j] = lo | (hi << 16);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < Q; ++j)
+      if (s.bit + j < p.valid_bits && s.off + j < p.ue_stride) dst[j] = v[j];
+  }
+}
+
+// Maps the hardware workgroup id to a logical work index so that consecutive
+// logical indices share an XCD (ids are dealt to the 8 XCDs round-robin).
+__device__ __forceinline__ uint32_t xcd_linear(uint32_t w, uint32_t T) {
+  const uint32_t q = T / 8, r = T % 8, x = w % 8, j = w / 8;
+  return x * q + min(x, r) + j;
+}
+
+// Waves per workgroup the register budget allows for an (MT, KS) tile.
+constexpr int max_waves(int MT, int KS) { return MT * KS <= 8 ? 16 : 8; }
+
+// W_g slice of wave (mg, wk) -> MFMA A operands. W[u + ld*b] (global memory,
+// ld = U, or an LDS copy with a padded ld).
+template <int MT, int KS>
+__device__ __forceinline__ void load_w_operands(const float2* W, uint32_t ld,
+                                                uint32_t U, uint32_t B,
+                                                uint32_t mg, uint32_t wk,
+                                                float (&wa1)[MT][KS],
+                                                float (&wa2)[MT][KS]) {
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const bool odd = i16 & 1;
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const uint32_t u = 8 * (mg * MT + m) + (i16 >> 1);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const uint32_t b = 4 * (wk * KS + s) + q4;
+      float2 v = make_float2(0.f, 0.f);
+      if (u < U && b < B) v = W[u + (uint64_t)ld * b];
+      wa1[m][s] = odd ? v.y : v.x;
+      wa2[m][s] = odd ? v.x : -v.y;
+    }
+  }
+}
+
+// Column tiles [t0, t1) of group g of `slot`, with this wave's W slice in
+// registers: GEMM, demod, LLR (and optional x_hat) stores. All waves of the
+// workgroup must call it with the same tile range (barriers when nwk > 1).
+template <int MT, int KS, int Q>
+__device__ __forceinline__ void eq_tiles(const EqParams& p,
+                                         const float (&wa1)[MT][KS],
+                                         const float (&wa2)[MT][KS],
+                                         uint32_t slot, uint32_t g,
+                                         uint32_t mg, uint32_t wk, uint32_t t0,
+                                         uint32_t t1, float* sRed) {
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const int nw = blockDim.x >> 6;
+  const uint32_t U = p.U, B = p.B;
+  const uint32_t k0 = g * p.sg;
+  const uint32_t nk = min(p.sg, p.N - k0);
+  const uint32_t ncols = nk * p.S;
+
+  const float2* Y = p.y + slot * p.y_slot + k0;
+  _Float16* L = p.llr + slot * p.llr_slot;
+  int buf = 0;
+
+  // y operands of tile t (zeros outside the group's columns / antennas)
+  auto load_y = [&](uint32_t t, float2 (&yv)[KS]) {
+    const uint32_t n = 16 * t + i16;
+    const bool valid_n = n < ncols;
+    const uint32_t sym = valid_n ? n / nk : 0;
+    const uint32_t kk = valid_n ? n - sym * nk : 0;
+    const float2* yc = Y + sym * p.y_sym + kk;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const uint32_t b = 4 * (wk * KS + s) + q4;
+      yv[s] = (valid_n && b < B) ? yc[(uint64_t)b * p.N] : make_float2(0.f, 0.f);
+    }
+  };
+
+  float2 ynext[KS];
+  load_y(t0, ynext);
+  for (uint32_t t = t0; t < t1; ++t) {
+    float2 yv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) yv[s] = ynext[s];
+    if (t + 1 < t1) load_y(t + 1, ynext);  // in flight during this tile's MFMAs
+
+    const uint32_t n = 16 * t + i16;
+    const bool valid_n = n < ncols;
+    const uint32_t sym = valid_n ? n / nk : 0;
+    const uint32_t kk = valid_n ? n - sym * nk : 0;
+
+    v4f acc1[MT], acc2[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      acc1[m] = v4f{0.f, 0.f, 0.f, 0.f};
+      acc2[m] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        acc1[m] = mfma4(wa1[m][s], yv[s].x, acc1[m]);
+        acc2[m] = mfma4(wa2[m][s], yv[s].y, acc2[m]);
+      }
+    }
+
+    const uint32_t k = k0 + kk;
+    const LlrSlot ls = llr_slot<Q>(p, k);
+    _Float16* rows = L + sym * p.llr_sym;
+    float2* xrow = p.xhat ? p.xhat + ((uint64_t)slot * p.S + sym) * U * p.N + k : nullptr;
+    auto epilogue = [&](int m, v4f d) {
+      if (!valid_n) return;
+      const uint32_t u0 = 8 * (mg * MT + m) + 2 * q4;
+      if (u0 < U) {
+        store_llrs<Q>(p, rows + u0 * p.ue_stride, ls, d[0], d[1]);
+        if (xrow) xrow[(uint64_t)u0 * p.N] = make_float2(d[0], d[1]);
+      }
+      if (u0 + 1 < U) {
+        store_llrs<Q>(p, rows + (u0 + 1) * p.ue_stride, ls, d[2], d[3]);
+        if (xrow) xrow[(uint64_t)(u0 + 1) * p.N] = make_float2(d[2], d[3]);
+      }
+    };
+
+    if (p.nwk == 1) {
+#pragma unroll
+      for (int m = 0; m < MT; ++m) epilogue(m, acc1[m] + acc2[m]);
+    } else {
+      float* mine = sRed + ((size_t)(buf * nw + wk) * MT) * 256;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const v4f d = acc1[m] + acc2[m];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) mine[m * 256 + r * 64 + lane] = d[r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        if (m % (int)p.nwk != (int)wk) continue;
+        v4f d = v4f{0.f, 0.f, 0.f, 0.f};
+        for (uint32_t w2 = 0; w2 < p.nwk; ++w2) {
+          const float* src = sRed + ((size_t)(buf * nw + w2) * MT + m) * 256;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) d[r] += src[r * 64 + lane];
+        }
+        epilogue(m, d);
+      }
+      buf ^= 1;
+    }
+  }
+}
+
+// Fills the launch-independent fields of EqParams from the C-ABI arguments
+// (equalize.hip).
+hipError_t fill_eq_params(const EqualizeArgs& a, EqParams& p, const char** why);
+
+// Host-side tile plan shared by the equalize and detect launchers.
+struct EqPlan {
+  uint32_t MT, KS, nmg, nwk;
+  size_t red_lds;  // bytes of the LDS reduction buffer (0 when nwk == 1)
+};
+
+inline uint32_t pow2ceil_u32(uint32_t v) {
+  uint32_t r = 1;
+  while (r < v) r <<= 1;
+  return r;
+}
+
+// MT M-tiles (8 users each) per wave, KS 4-antenna steps per wave; the W slice
+// per wave is 2*MT*KS registers (<= 64, so that two tiles of y operands, the
+// accumulators and addresses fit without scratch at 2 waves per SIMD).
+inline EqPlan plan_equalize(uint32_t U, uint32_t B, uint32_t ks_override) {
+  EqPlan pl;
+  pl.MT = U <= 8 ? 1 : U <= 16 ? 2 : 4;
+  pl.nmg = (U + 8 * pl.MT - 1) / (8 * pl.MT);
+  const uint32_t cap = pl.MT == 4 ? 8u : 16u;
+  uint32_t KS = pow2ceil_u32((B + 3) / 4);
+  KS = KS < 8 ? 8 : KS;
+  KS = KS > cap ? cap : KS;
+  if (ks_override == 8 || (ks_override == 16 && pl.MT <= 2)) KS = ks_override;
+  pl.KS = KS;
+  pl.nwk = (B + 4 * KS - 1) / (4 * KS);
+  pl.red_lds = pl.nwk > 1 ? (size_t)2 * pl.nwk * pl.MT * 256 * sizeof(float) : 0;
+  return pl;
+}
+
+}  // namespace detail
+}  // namespace mega

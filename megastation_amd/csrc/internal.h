@@ -28,6 +28,13 @@ hipError_t launch_zf(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
                      float2* out, int32_t* info, hipStream_t stream,
                      const char** why);
 
+// ZF + equalize + demod of every symbol of every slot: fused in one kernel for
+// U <= 16 and B <= 256 (w_out optional), otherwise the ZF and equalize kernels
+// back to back through w_out (required then).
+hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
+                         hipStream_t stream, const char** why);
+bool detect_fused_supported(uint32_t U, uint32_t B);
+
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,
                                hipStream_t stream);

@@ -247,3 +247,89 @@ def test_detect_end_to_end_evm(U, B, N, sg, Q, channel, snr):
         e_gpu = synth.evm_db(xh[0, s], sl.x[s])
         e_64 = synth.evm_db(x64, sl.x[s])
         assert abs(e_gpu - e_64) < 0.1, (e_gpu, e_64)
+
+
+# ----------------------------------------------------------------------------
+# fused detection (mega_detect_slots)
+# ----------------------------------------------------------------------------
+
+
+def _detect(csi, y, U, B, N, sg, Q, lay, slots, S, want_w=True, want_x=True):
+    t = torch()
+    G = (N + sg - 1) // sg
+    d_a, d_y = dev_c64(csi), dev_c64(y)
+    llr = llr_buffer(slots * S * U * lay.ue_stride)
+    w = t.zeros_like(d_a) if want_w else None
+    xh = t.zeros(slots * S * U * N, dtype=t.complex64, device="cuda:0") if want_x else None
+    info = t.full((slots * G,), -5, dtype=t.int32, device="cuda:0")
+    m.detect_slots(d_a, d_y, llr, num_ue=U, num_bs=B, num_sc=N, sc_group=sg,
+                   num_syms=S, num_slots=slots, mod_order=Q, ue_stride=lay.ue_stride,
+                   cb_size=lay.cb_size, cb_stride=lay.cb_stride, zeros=lay.zeros,
+                   valid_bits=lay.valid_bits, csi_slot_stride=G * B * U,
+                   y_sym_stride=B * N, y_slot_stride=S * B * N,
+                   llr_sym_stride=U * lay.ue_stride,
+                   llr_slot_stride=S * U * lay.ue_stride, xhat=xh, w_out=w, info=info)
+    _sync()
+    return (host_u16(llr).reshape(slots, S, U, lay.ue_stride),
+            host_c64(xh).reshape(slots, S, U, N) if want_x else None,
+            host_c64(w).reshape(slots, G, B, U) if want_w else None,
+            info.cpu().numpy().reshape(slots, G))
+
+
+@pytest.mark.parametrize("U,B,N,sg,Q", [(16, 64, 1200, 16, 4), (8, 64, 624, 12, 8),
+                                        (16, 128, 600, 16, 6), (12, 192, 300, 12, 2),
+                                        (16, 256, 240, 16, 4), (5, 24, 100, 7, 6),
+                                        (32, 128, 480, 16, 6)])
+def test_detect_slots_equals_zf_then_equalize(U, B, N, sg, Q):
+    """Fused kernel == ZF kernel followed by the equalize kernel, bit for bit
+    (same arithmetic), and x_hat / EVM against the FP64 oracle."""
+    print(f"case: detect U={U} B={B} N={N} sg={sg} Q={Q}", flush=True)
+    S, slots = 3, 2
+    sl = make_slots(U * 31 + B, slots, U, B, N, sg, S, Q, 25.0)
+    lay = synth.standalone_layout(N, Q)
+    G = (N + sg - 1) // sg
+    csi = np.stack([s.csi for s in sl])
+    y = np.stack([s.y for s in sl])
+    llr_f, x_f, w_f, info = _detect(csi, y, U, B, N, sg, Q, lay, slots, S)
+    assert not info.any()
+    t = torch()
+    d_w = t.zeros(slots * G * B * U, dtype=t.complex64, device="cuda:0")
+    m.zf(dev_c64(csi), d_w, B, U, slots * G)
+    _sync()
+    w_sep = host_c64(d_w).reshape(slots, G, B, U)
+    assert np.array_equal(w_f, w_sep)
+    llr_s, x_s = gpu_equalize_slots(w_sep, y, U, B, N, sg, Q, lay, slots=slots, S=S)
+    assert np.array_equal(llr_f, llr_s)
+    assert np.array_equal(x_f, x_s)
+    w64, _, _ = oracle.zf(csi[0], U, B, G, "f64")
+    for s in range(S):
+        llr64, x64 = oracle_symbol_f64(w64, y[0, s], U, B, N, sg, Q, lay)
+        check_symbol(llr_f[0, s], x_f[0, s], llr64, x64, N, sg, Q, lay)
+        assert abs(synth.evm_db(x_f[0, s], sl[0].x[s]) - synth.evm_db(x64, sl[0].x[s])) < 0.1
+
+
+def test_detect_slots_failed_group_and_no_w_out():
+    print("case: detect failed group, no w_out", flush=True)
+    U, B, N, sg, Q, S = 8, 32, 64, 16, 4, 2
+    sl = make_slots(3, 1, U, B, N, sg, S, Q)
+    csi = sl[0].csi[None].copy()
+    csi[0, 2] = 0                              # group 2: no channel -> singular
+    lay = synth.standalone_layout(N, Q)
+    llr, xh, _, info = _detect(csi, sl[0].y[None], U, B, N, sg, Q, lay, 1, S,
+                               want_w=False)
+    assert info[0, 2] == 1 and not np.delete(info[0], 2).any()
+    assert not xh[0, :, :, 32:48].any()        # W = 0 -> x_hat = 0 for its subcarriers
+    z = oracle.demod(np.zeros(1, np.complex64), Q).reshape(-1)
+    for u in range(U):
+        for k in range(32, 48):
+            assert np.array_equal(llr[0, 0, u, k * Q:(k + 1) * Q], z)
+
+
+def test_detect_slots_requires_w_out_outside_fused_shapes():
+    print("case: detect U=32 without w_out", flush=True)
+    U, B, N, sg, Q = 32, 64, 48, 16, 4
+    sl = make_slots(1, 1, U, B, N, sg, 1, Q)
+    lay = synth.standalone_layout(N, Q)
+    with pytest.raises(m.MegaError) as e:
+        _detect(sl[0].csi[None], sl[0].y[None], U, B, N, sg, Q, lay, 1, 1, want_w=False)
+    assert e.value.code == m.MEGA_ERR_INVALID_ARG
