@@ -44,6 +44,26 @@ def _compile(src: str) -> str:
     return obj
 
 
+CPP_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_dropin.cpp")
+CPP_TEST_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_dropin")
+
+
+def build_cpp_tests(verbose: bool = True) -> str:
+    """The C++ drop-in test program (include/mega/ classes over the C ABI)."""
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           f"-I{os.path.join(os.path.dirname(HERE), 'include')}", CPP_TEST_SRC,
+           f"-L{LIBDIR}", "-lmegadetect", "-Wl,-rpath,$ORIGIN/../../megastation_amd/lib",
+           "-o", CPP_TEST_BIN + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for test_dropin.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(CPP_TEST_BIN + ".tmp", CPP_TEST_BIN)
+    if verbose:
+        print("built", CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -63,3 +83,4 @@ def build(verbose: bool = True) -> str:
 
 if __name__ == "__main__":
     build()
+    build_cpp_tests()

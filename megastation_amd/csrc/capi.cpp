@@ -8,7 +8,7 @@
 #include <string>
 
 #include "../../include/megadetect.h"
-#include "demod.h"
+#include "mega/demod_core.h"
 #include "internal.h"
 
 namespace {

@@ -22,7 +22,7 @@
 // tile).
 #pragma once
 
-#include "demod.h"
+#include "mega/demod_core.h"
 #include "internal.h"
 
 namespace mega {
