@@ -187,6 +187,8 @@ def main():
         dom_kernel, dom_ms = "equalize_demod", eq_ms
     value = symbols / (ms_step * 1e-3)
     achieved = io_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, "detect_fused" if head_form == "fused"
+                               else "equalize_mfma")
 
     # ---- EVM of the headline form on slot 0 (x_hat side output, untimed) ----
     xh = torch.zeros(S * U * N, dtype=torch.complex64, device=dev)
@@ -216,7 +218,9 @@ def main():
                    "parallelism": "1 GPU, groups x slots"},
         "roofline": {"bound": "hbm", "kernel": dom_kernel,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["traffic_bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "algorithmic_bytes": io_bytes, "avg_ms": dom_ms},
         "forms": {
             "fused": ({"symbols_per_s": symbols / (forms["fused"][0] * 1e-3),
@@ -237,6 +241,23 @@ def main():
         result["evm"]["delta_db"] = evm_gpu - evm64
         result["evm"]["within_0p1_db"] = abs(evm_gpu - evm64) < 0.1
     print(json.dumps(result), flush=True)
+
+
+def measured_traffic(workload, kernel_prefix):
+    """HBM bytes per launch of the dominant kernel from the last rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this workload (copy-calibrated), recorded
+    in profiles/traffic.json by megastation_amd.profiling; None if absent."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        data = json.load(fh).get(workload, {})
+    for k, v in data.items():
+        if k.startswith(kernel_prefix):
+            return {"traffic_bytes": v["traffic_bytes"],
+                    "source": f"profiles/traffic.json: {k}, rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                              "scaled by a 1 GiB copy"}
+    return None
 
 
 def cpu_baseline(cfg, lay, csi, y, x, budget_s):
