@@ -14,21 +14,35 @@
 // Shapes: up to 16 users (one 16-row Gram tile) and 256 antennas (four waves
 // of 64 antennas); other shapes run the two kernels back to back through
 // w_out.
+#include <cstdlib>
+
 #include "zf_core.h"
 
 namespace mega {
 namespace detail {
 namespace {
 
-constexpr uint32_t kLdw = 24;  // LDS row of W (complex): conflict-free operand reads
+// Dynamic LDS: A staged as rows of U+1 complex (W is written over it and
+// read from there by the equalize), then one region used first for the Gram
+// partials and afterwards for the equalize reduction.
+template <int NW>
+size_t detect_lds(uint32_t U, uint32_t B, uint32_t MT) {
+  const size_t a = (size_t)(U + 1) * B * sizeof(float2);
+  const size_t part = zf_dyn_lds<1, 64 * NW>(U, B, false);
+  const size_t red = NW > 1 ? (size_t)2 * NW * MT * 256 * sizeof(float) : 0;
+  return a + (part > red ? part : red);
+}
 
-template <int MT, int KS, int NW, int Q>
+template <int MT, int KS, int NW, int Q, bool PF>
 __global__ __launch_bounds__(64 * NW) void detect_fused(const EqParams p) {
   __shared__ ZfShared<1> sh;
   extern __shared__ float sDyn[];
-  float2* sW = reinterpret_cast<float2*>(sDyn);
-  float* sRed = sDyn + 2 * kLdw * p.B;
+  float2* sA = reinterpret_cast<float2*>(sDyn);
+  float* region = sDyn + 2 * (p.U + 1) * p.B;
+  float2* part = reinterpret_cast<float2*>(region);
+  float* sRed = region;  // reused after the ZF phase
   const uint32_t wk = threadIdx.x >> 6;
+  const uint32_t ldw = p.U + 1;
 
   const uint32_t logical = xcd_linear(blockIdx.x, gridDim.x);
   const uint32_t g = logical % p.G;
@@ -36,37 +50,53 @@ __global__ __launch_bounds__(64 * NW) void detect_fused(const EqParams p) {
   const uint64_t goff = slot * p.csi_slot + (uint64_t)g * p.U * p.B;
   float2* wout = p.w_out ? p.w_out + goff : nullptr;
 
-  const int fail = zf_group<1, 64 * NW>(p.csi + goff, p.U, p.B, sh, sW, kLdw, wout);
+  // The first y tile does not depend on W: issue it before the ZF phase.
+  float2 ynext[KS];
+  eq_load_tile<KS>(p, slot, g, wk, 0, ynext);
+
+  // W overwrites A in LDS (each wave owns whole antenna columns of both).
+  const int fail =
+      zf_group<1, 64 * NW, true>(p.csi + goff, p.U, p.B, sh, sA, part, sA, ldw, wout);
   if (threadIdx.x == 0 && p.info) p.info[(uint64_t)slot * p.G + g] = fail;
   __syncthreads();
 
-  float wa1[MT][KS], wa2[MT][KS];
-  load_w_operands<MT, KS>(sW, kLdw, p.U, p.B, 0, wk, wa1, wa2);
+  const WLds<MT, KS> ws{sA, ldw, p.U, p.B, 0, wk};
   const uint32_t nk = min(p.sg, p.N - g * p.sg);
   const uint32_t ntiles = (nk * p.S + 15) / 16;
-  eq_tiles<MT, KS, Q>(p, wa1, wa2, slot, g, 0, wk, 0, ntiles, sRed);
+  eq_tiles<MT, KS, Q, WLds<MT, KS>, PF>(p, ws, slot, g, 0, wk, 0, ntiles, sRed, ynext);
 }
 
-template <int MT, int NW, int Q>
+template <int MT, int NW, int Q, bool PF>
 hipError_t launch_one(const EqParams& p, dim3 grid, size_t lds, hipStream_t st) {
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&detect_fused<MT, 16, NW, Q>),
+        reinterpret_cast<const void*>(&detect_fused<MT, 16, NW, Q, PF>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((detect_fused<MT, 16, NW, Q>), grid, dim3(64 * NW), lds, st, p);
+  hipLaunchKernelGGL((detect_fused<MT, 16, NW, Q, PF>), grid, dim3(64 * NW), lds, st, p);
   return hipGetLastError();
+}
+
+// MEGA_DET_PF=0 turns the next-tile y prefetch off (tuning sweeps).
+bool detect_prefetch() {
+  const char* e = std::getenv("MEGA_DET_PF");
+  return !(e && e[0] == '0');
 }
 
 template <int MT, int NW>
 hipError_t launch_q(const EqParams& p, dim3 grid, size_t lds, uint32_t Q,
                     hipStream_t st) {
-  switch (Q) {
-    case 2: return launch_one<MT, NW, 2>(p, grid, lds, st);
-    case 4: return launch_one<MT, NW, 4>(p, grid, lds, st);
-    case 6: return launch_one<MT, NW, 6>(p, grid, lds, st);
-    case 8: return launch_one<MT, NW, 8>(p, grid, lds, st);
+  const bool pf = detect_prefetch();
+  switch (Q * 2 + (pf ? 1 : 0)) {
+    case 5: return launch_one<MT, NW, 2, true>(p, grid, lds, st);
+    case 4: return launch_one<MT, NW, 2, false>(p, grid, lds, st);
+    case 9: return launch_one<MT, NW, 4, true>(p, grid, lds, st);
+    case 8: return launch_one<MT, NW, 4, false>(p, grid, lds, st);
+    case 13: return launch_one<MT, NW, 6, true>(p, grid, lds, st);
+    case 12: return launch_one<MT, NW, 6, false>(p, grid, lds, st);
+    case 17: return launch_one<MT, NW, 8, true>(p, grid, lds, st);
+    case 16: return launch_one<MT, NW, 8, false>(p, grid, lds, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -109,8 +139,8 @@ hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
     *why = "grid too large";
     return hipErrorInvalidValue;
   }
-  const size_t lds = (size_t)kLdw * a.B * sizeof(float2) +
-                     (nw > 1 ? (size_t)2 * nw * MT * 256 * sizeof(float) : 0);
+  const size_t lds = nw == 1 ? detect_lds<1>(a.U, a.B, MT)
+                    : nw == 2 ? detect_lds<2>(a.U, a.B, MT) : detect_lds<4>(a.U, a.B, MT);
   dim3 grid((uint32_t)nblocks);
   switch (MT * 10 + nw) {
     case 11: return launch_q<1, 1>(p, grid, lds, a.Q, st);

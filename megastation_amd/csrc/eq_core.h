@@ -101,74 +101,108 @@ __device__ __forceinline__ uint32_t xcd_linear(uint32_t w, uint32_t T) {
 // Waves per workgroup the register budget allows for an (MT, KS) tile.
 constexpr int max_waves(int MT, int KS) { return MT * KS <= 8 ? 16 : 8; }
 
-// W_g slice of wave (mg, wk) -> MFMA A operands. W[u + ld*b] (global memory,
-// ld = U, or an LDS copy with a padded ld).
+// MFMA A operands of wave (mg, wk)'s slice of W_g, for users
+// u = 8*(mg*MT + m) + (lane%16)/2 and antennas b = 4*(wk*KS + s) + lane/16:
+//   a1 = lane odd ? Im W[u,b] : Re W[u,b],   a2 = lane odd ? Re W[u,b] : -Im W[u,b]
+// (zero outside U x B). Two sources with the same interface:
+//   WRegs  - the slice held in 2*MT*KS registers, loaded once;
+//   WLds   - read per use from a copy of W in LDS (W[u + ld*b]), which frees
+//            those registers for occupancy.
 template <int MT, int KS>
-__device__ __forceinline__ void load_w_operands(const float2* W, uint32_t ld,
-                                                uint32_t U, uint32_t B,
-                                                uint32_t mg, uint32_t wk,
-                                                float (&wa1)[MT][KS],
-                                                float (&wa2)[MT][KS]) {
+struct WRegs {
+  float a1[MT][KS], a2[MT][KS];
+  __device__ __forceinline__ void load(const float2* W, uint32_t ld, uint32_t U,
+                                       uint32_t B, uint32_t mg, uint32_t wk) {
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, q4 = lane >> 4;
+    const bool odd = i16 & 1;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const uint32_t u = 8 * (mg * MT + m) + (i16 >> 1);
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const uint32_t b = 4 * (wk * KS + s) + q4;
+        float2 v = make_float2(0.f, 0.f);
+        if (u < U && b < B) v = W[u + (uint64_t)ld * b];
+        a1[m][s] = odd ? v.y : v.x;
+        a2[m][s] = odd ? v.x : -v.y;
+      }
+    }
+  }
+  __device__ __forceinline__ void get(int m, int s, float& x1, float& x2) const {
+    x1 = a1[m][s];
+    x2 = a2[m][s];
+  }
+};
+
+template <int MT, int KS>
+struct WLds {
+  const float2* w;
+  uint32_t ld, U, B, mg, wk;
+  __device__ __forceinline__ void get(int m, int s, float& x1, float& x2) const {
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, q4 = lane >> 4;
+    const uint32_t u = 8 * (mg * MT + m) + (i16 >> 1);
+    const uint32_t b = 4 * (wk * KS + s) + q4;
+    const float2 v = (u < U && b < B) ? w[u + ld * b] : make_float2(0.f, 0.f);
+    const bool odd = i16 & 1;
+    x1 = odd ? v.y : v.x;
+    x2 = odd ? v.x : -v.y;
+  }
+};
+
+// y operands of column tile t of group g (zeros outside the group's columns
+// and beyond B): lane = column, 16 lanes read 128 contiguous bytes of a row.
+template <int KS>
+__device__ __forceinline__ void eq_load_tile(const EqParams& p, uint32_t slot,
+                                             uint32_t g, uint32_t wk, uint32_t t,
+                                             float2 (&yv)[KS]) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, q4 = lane >> 4;
-  const bool odd = i16 & 1;
+  const uint32_t k0 = g * p.sg;
+  const uint32_t nk = min(p.sg, p.N - k0);
+  const uint32_t n = 16 * t + i16;
+  const bool valid_n = n < nk * p.S;
+  const uint32_t sym = valid_n ? n / nk : 0;
+  const uint32_t kk = valid_n ? n - sym * nk : 0;
+  const float2* yc = p.y + slot * p.y_slot + k0 + sym * p.y_sym + kk;
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const uint32_t u = 8 * (mg * MT + m) + (i16 >> 1);
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const uint32_t b = 4 * (wk * KS + s) + q4;
-      float2 v = make_float2(0.f, 0.f);
-      if (u < U && b < B) v = W[u + (uint64_t)ld * b];
-      wa1[m][s] = odd ? v.y : v.x;
-      wa2[m][s] = odd ? v.x : -v.y;
-    }
+  for (int s = 0; s < KS; ++s) {
+    const uint32_t b = 4 * (wk * KS + s) + q4;
+    yv[s] = (valid_n && b < p.B) ? yc[(uint64_t)b * p.N] : make_float2(0.f, 0.f);
   }
 }
 
-// Column tiles [t0, t1) of group g of `slot`, with this wave's W slice in
-// registers: GEMM, demod, LLR (and optional x_hat) stores. All waves of the
-// workgroup must call it with the same tile range (barriers when nwk > 1).
-template <int MT, int KS, int Q>
-__device__ __forceinline__ void eq_tiles(const EqParams& p,
-                                         const float (&wa1)[MT][KS],
-                                         const float (&wa2)[MT][KS],
+// Column tiles [t0, t1) of group g of `slot`, with this wave's W slice from
+// ws and tile t0's y operands already issued into ynext: GEMM, demod,
+// LLR (and optional x_hat) stores. All waves of the workgroup must call it
+// with the same tile range (barriers when nwk > 1).
+template <int MT, int KS, int Q, class WSrc, bool PREFETCH = true>
+__device__ __forceinline__ void eq_tiles(const EqParams& p, const WSrc& ws,
                                          uint32_t slot, uint32_t g,
                                          uint32_t mg, uint32_t wk, uint32_t t0,
-                                         uint32_t t1, float* sRed) {
+                                         uint32_t t1, float* sRed,
+                                         float2 (&ynext)[KS]) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, q4 = lane >> 4;
   const int nw = blockDim.x >> 6;
-  const uint32_t U = p.U, B = p.B;
+  const uint32_t U = p.U;
   const uint32_t k0 = g * p.sg;
   const uint32_t nk = min(p.sg, p.N - k0);
   const uint32_t ncols = nk * p.S;
-
-  const float2* Y = p.y + slot * p.y_slot + k0;
   _Float16* L = p.llr + slot * p.llr_slot;
   int buf = 0;
 
-  // y operands of tile t (zeros outside the group's columns / antennas)
-  auto load_y = [&](uint32_t t, float2 (&yv)[KS]) {
-    const uint32_t n = 16 * t + i16;
-    const bool valid_n = n < ncols;
-    const uint32_t sym = valid_n ? n / nk : 0;
-    const uint32_t kk = valid_n ? n - sym * nk : 0;
-    const float2* yc = Y + sym * p.y_sym + kk;
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const uint32_t b = 4 * (wk * KS + s) + q4;
-      yv[s] = (valid_n && b < B) ? yc[(uint64_t)b * p.N] : make_float2(0.f, 0.f);
-    }
-  };
-
-  float2 ynext[KS];
-  load_y(t0, ynext);
   for (uint32_t t = t0; t < t1; ++t) {
     float2 yv[KS];
+    if (PREFETCH || t == t0) {
 #pragma unroll
-    for (int s = 0; s < KS; ++s) yv[s] = ynext[s];
-    if (t + 1 < t1) load_y(t + 1, ynext);  // in flight during this tile's MFMAs
+      for (int s = 0; s < KS; ++s) yv[s] = ynext[s];
+    } else {
+      eq_load_tile<KS>(p, slot, g, wk, t, yv);
+    }
+    if (PREFETCH && t + 1 < t1)
+      eq_load_tile<KS>(p, slot, g, wk, t + 1, ynext);  // in flight during the MFMAs
 
     const uint32_t n = 16 * t + i16;
     const bool valid_n = n < ncols;
@@ -185,8 +219,10 @@ __device__ __forceinline__ void eq_tiles(const EqParams& p,
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
       for (int m = 0; m < MT; ++m) {
-        acc1[m] = mfma4(wa1[m][s], yv[s].x, acc1[m]);
-        acc2[m] = mfma4(wa2[m][s], yv[s].y, acc2[m]);
+        float x1, x2;
+        ws.get(m, s, x1, x2);
+        acc1[m] = mfma4(x1, yv[s].x, acc1[m]);
+        acc2[m] = mfma4(x2, yv[s].y, acc2[m]);
       }
     }
 

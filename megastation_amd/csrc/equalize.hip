@@ -41,10 +41,12 @@ __global__ __launch_bounds__(64 * max_waves(MT, KS)) void equalize_mfma(const Eq
   const uint32_t t1 = min(t0 + p.tiles_per_wg, ntiles);
   if (t0 >= t1) return;  // uniform over the workgroup
 
-  float wa1[MT][KS], wa2[MT][KS];
   const float2* W = p.csi + slot * p.csi_slot + (uint64_t)g * p.U * p.B;
-  load_w_operands<MT, KS>(W, p.U, p.U, p.B, mg, wk, wa1, wa2);
-  eq_tiles<MT, KS, Q>(p, wa1, wa2, slot, g, mg, wk, t0, t1, sRed);
+  float2 ynext[KS];
+  eq_load_tile<KS>(p, slot, g, wk, t0, ynext);
+  WRegs<MT, KS> ws;
+  ws.load(W, p.U, p.U, p.B, mg, wk);
+  eq_tiles<MT, KS, Q>(p, ws, slot, g, mg, wk, t0, t1, sRed, ynext);
 }
 
 template <int MT, int KS, int Q>

@@ -10,14 +10,39 @@ namespace {
 template <int UT>
 constexpr int zf_threads() { return UT == 1 ? 64 : 256; }
 
-template <int UT>  // U <= 16 * UT
+template <int UT, bool A_LDS>  // U <= 16 * UT
 __global__ __launch_bounds__(zf_threads<UT>()) void zf_mfma(
     const float2* in, float2* out, int32_t* info, uint32_t U, uint32_t B) {
   __shared__ ZfShared<UT> sh;
+  extern __shared__ float sDyn[];
+  float2* sA = reinterpret_cast<float2*>(sDyn);
+  float2* part = reinterpret_cast<float2*>(sDyn + (A_LDS ? 2 * (U + 1) * B : 0));
   const uint32_t g = blockIdx.x;
-  const int fail = zf_group<UT, zf_threads<UT>()>(in + (uint64_t)g * U * B, U, B, sh,
-                                                   out + (uint64_t)g * U * B, U, nullptr);
+  const int fail = zf_group<UT, zf_threads<UT>(), A_LDS>(
+      in + (uint64_t)g * U * B, U, B, sh, sA, part, out + (uint64_t)g * U * B, U, nullptr);
   if (threadIdx.x == 0 && info) info[g] = fail;
+}
+
+// Static LDS of ZfShared<UT> plus the dynamic part must stay within the
+// 160 KB a workgroup may use; A is staged only when that holds.
+template <int UT>
+hipError_t launch_zf_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                        float2* out, int32_t* info, hipStream_t st) {
+  constexpr int NTH = zf_threads<UT>();
+  const size_t with_a = zf_dyn_lds<UT, NTH>(U, B, true);
+  const bool a_lds = sizeof(ZfShared<UT>) + with_a <= 120 * 1024;
+  const size_t lds = a_lds ? with_a : zf_dyn_lds<UT, NTH>(U, B, false);
+  const void* fn = a_lds ? reinterpret_cast<const void*>(&zf_mfma<UT, true>)
+                         : reinterpret_cast<const void*>(&zf_mfma<UT, false>);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  if (a_lds)
+    hipLaunchKernelGGL((zf_mfma<UT, true>), dim3(batch), dim3(NTH), lds, st, in, out, info, U, B);
+  else
+    hipLaunchKernelGGL((zf_mfma<UT, false>), dim3(batch), dim3(NTH), lds, st, in, out, info, U, B);
+  return hipGetLastError();
 }
 
 // P_g = W_g^T * (1 / max|W_g|) (beam_utils.cuh:40-80): U x B column-major in,
@@ -60,18 +85,11 @@ hipError_t launch_zf(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
     *why = "batch too large";
     return hipErrorInvalidValue;
   }
-  dim3 grid(batch);
-  if (U <= 16)
-    hipLaunchKernelGGL(zf_mfma<1>, grid, dim3(zf_threads<1>()), 0, st, in, out, info, U, B);
-  else if (U <= 32)
-    hipLaunchKernelGGL(zf_mfma<2>, grid, dim3(zf_threads<2>()), 0, st, in, out, info, U, B);
-  else if (U <= 64)
-    hipLaunchKernelGGL(zf_mfma<4>, grid, dim3(zf_threads<4>()), 0, st, in, out, info, U, B);
-  else {
-    *why = "more than 64 users";
-    return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  if (U <= 16) return launch_zf_ut<1>(U, B, batch, in, out, info, st);
+  if (U <= 32) return launch_zf_ut<2>(U, B, batch, in, out, info, st);
+  if (U <= 64) return launch_zf_ut<4>(U, B, batch, in, out, info, st);
+  *why = "more than 64 users";
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,

@@ -3,23 +3,32 @@
 //
 // Reference: src/operators/beamform/beamform.cu:74-122 runs a batched Cgemm
 // (Gram), a batched potrf (lower Cholesky) and G*B single-right-hand-side
-// potrs solves through cuBLAS/cuSOLVER. Here the three steps are fused per
-// group, both products on the FP32 matrix cores (v_mfma_f32_16x16x4_f32):
+// potrs solves through cuBLAS/cuSOLVER. Here the steps are fused per group,
+// both products on the FP32 matrix cores (v_mfma_f32_16x16x4_f32):
+//   0. A is staged into LDS once (rows of U+1 complex, all loads in flight)
+//      when it fits; both products then read it from there.
 //   1. Gram G = A A^H, users padded to 16-row tiles, K = 4 antennas per step:
 //        Re G += Re A Re A^T + Im A Im A^T,  Im G += Im A Re A^T - Re A Im A^T.
 //      For the 16x16x4 instruction the A operand of lane l is X[l%16][l/16]
 //      and the B operand is Y[l%16][l/16] for C = X Y^T, so one load of
-//      A[u = l%16][b = 4s + l/16] (64 contiguous complex for U = 16) feeds
-//      both operands of a diagonal tile.
-//   2. G^-1 by in-place Gauss-Jordan elimination in LDS (no pivoting: G is
-//      Hermitian positive definite, and its pivots are exactly the squared
-//      Cholesky diagonals). A pivot not above U * 2^-23 * G_kk marks the group
-//      as numerically singular: status k+1 and W = 0 (the reference never
+//      A[u = l%16][b = 4s + l/16] feeds both operands of a diagonal tile.
+//      The antennas are summed in fixed chunks of 64 (partial sums added in
+//      chunk order), so the result does not depend on how many waves share
+//      the work.
+//   2. -G^-1 by U sweeps of the (Hermitian-preserving) sweep operator on the
+//      Gram matrix: for pivot k, with a'_ik = -1 when i = k and a'_kc = -1
+//      when c = k,
+//        new[i][c] = (i == k or c == k ? 0 : a[i][c]) - a'_ik * a'_kc / a_kk,
+//      which after all pivots leaves -G^-1 (no pivoting: G is Hermitian
+//      positive definite and the pivots are exactly the squared Cholesky
+//      diagonals). For up to 16 users one wave does it in registers with
+//      cross-lane shuffles; larger groups ping-pong between two LDS copies
+//      (one barrier per pivot). A pivot not above U * 2^-23 * G_kk marks the
+//      group numerically singular: status k+1 and W = 0 (the reference never
 //      reads cuSOLVER's status and would emit Inf/NaN).
-//   3. W = G^-1 A, M = users, N = 16 antennas, K = 4 users per step:
-//        Re W = Re Gi Re A - Im Gi Im A,  Im W = Re Gi Im A + Im Gi Re A.
-//      A wave owns whole antenna columns (all user tiles), loads them once
-//      and only then stores, so writing W over A (in place) is safe.
+//   3. W = G^-1 A = -(sweep result) A, M = users, N = 16 antennas, K = 4 users
+//      per step. A wave owns whole antenna columns (all user tiles), loads
+//      them before storing, so writing W over A (in place) is safe.
 #pragma once
 
 #include "eq_core.h"
@@ -33,55 +42,178 @@ template <int UT>
 struct ZfShared {
   static constexpr int UP = 16 * UT;
   static constexpr int LDG = UP + 1;  // padded row (complex elements)
-  float2 G[UP * LDG];
+  static constexpr int NBUF = UT == 1 ? 1 : 2;
+  float2 G[NBUF][UP * LDG];
   float diag[UP];
 };
 
-// Returns the group status (0 = ok, k+1 = pivot k failed), identical in every
-// thread. W is written as W[u + ld_w * b] (ld_w = U for the CSI layout, or a
-// padded LDS row); w2 (optional) receives a second copy with ld = U.
+// Gram work split: NT lower-triangle tiles x NCH antenna chunks of 64. When a
+// workgroup has more waves than tiles, the chunks of a tile are spread over
+// the waves and their partial sums go through LDS; otherwise a wave sums its
+// tile's chunks in registers, in the same order.
+template <int UT, int NW>
+struct ZfPlan {
+  static constexpr int NT = UT * (UT + 1) / 2;
+  static constexpr bool SPLIT = NW > NT;
+};
+
+constexpr uint32_t kGramChunk = 16;  // K steps (4 antennas each) per chunk
+
+__host__ __device__ constexpr uint32_t gram_chunks(uint32_t B) {
+  return ((B + 3) / 4 + kGramChunk - 1) / kGramChunk;
+}
+
+// LDS bytes zf_group needs beyond ZfShared: A (when staged) and Gram partials.
 template <int UT, int NTH>
+__host__ __device__ constexpr size_t zf_dyn_lds(uint32_t U, uint32_t B, bool a_lds) {
+  return (a_lds ? (size_t)(U + 1) * B * sizeof(float2) : 0) +
+         (ZfPlan<UT, NTH / 64>::SPLIT
+              ? (size_t)ZfPlan<UT, NTH / 64>::NT * gram_chunks(B) * 256 * sizeof(float2)
+              : 0);
+}
+
+// Copies A (U x B, column-major) into LDS rows of U+1 complex, with every
+// thread keeping up to 16 loads in flight.
+template <int NTH>
+__device__ __forceinline__ void stage_a(const float2* A, uint32_t U, uint32_t B,
+                                        float2* sA) {
+  const uint32_t total = U * B;
+  const int tid = threadIdx.x;
+  for (uint32_t base = 0; base < total; base += NTH * 16) {
+    float2 tmp[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t e = base + j * NTH + tid;
+      tmp[j] = e < total ? A[e] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t e = base + j * NTH + tid;
+      if (e < total) {
+        const uint32_t b = e / U, u = e - b * U;
+        sA[b * (U + 1) + u] = tmp[j];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_of(int tile, int& mi, int& mj) {
+  mi = 0;
+  mj = tile;
+  while (mj > mi) {
+    mj -= mi + 1;
+    ++mi;
+  }
+}
+
+__device__ __forceinline__ float2 cmul2(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// One sweep-operator update of entry (i, c) for pivot k (see the file head).
+__device__ __forceinline__ float2 sweep_entry(float2 aic, float2 aik, float2 akc,
+                                              int i, int c, int k, float ip) {
+  const float2 base = (i == k || c == k) ? make_float2(0.f, 0.f) : aic;
+  const float2 x = (i == k) ? make_float2(-1.f, 0.f) : aik;
+  const float2 y = (c == k) ? make_float2(-1.f, 0.f) : akc;
+  const float2 p = cmul2(x, y);
+  return make_float2(base.x - p.x * ip, base.y - p.y * ip);
+}
+
+// Returns the group status (0 = ok, k+1 = pivot k failed), identical in every
+// thread. A is read from global memory (ld = U) or, when A_LDS, first staged
+// into sA (rows of U+1). W is written as W[u + ld_w * b] (ld_w = U for the
+// CSI layout, or a padded LDS row); w2 (optional) receives a second copy with
+// ld = U. part: LDS scratch for split Gram partials (zf_dyn_lds).
+template <int UT, int NTH, bool A_LDS>
 __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
-                                        ZfShared<UT>& sh, float2* w,
-                                        uint32_t ld_w, float2* w2) {
+                                        ZfShared<UT>& sh, float2* sA, float2* part,
+                                        float2* w, uint32_t ld_w, float2* w2) {
   constexpr int UP = ZfShared<UT>::UP;
   constexpr int LDG = ZfShared<UT>::LDG;
   constexpr int NW = NTH / 64;
-  constexpr int EPT = (UP * UP + NTH - 1) / NTH;
-  float2* sG = sh.G;
+  using Plan = ZfPlan<UT, NW>;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int i16 = lane & 15, q4 = lane >> 4;
   const uint32_t ks = (B + 3) / 4;
+  const uint32_t nch = gram_chunks(B);
+  const uint32_t lda = A_LDS ? U + 1 : U;
+  const float2* Asrc = A_LDS ? sA : A;
+  float2* sG = sh.G[0];
 
-  // ---- 1. Gram tiles (mi >= mj) ----------------------------------------------
-  constexpr int NT = UT * (UT + 1) / 2;
-  for (int tile = wave; tile < NT; tile += NW) {
-    int mi = 0, mj = tile;
-    while (mj > mi) {
-      mj -= mi + 1;
-      ++mi;
-    }
+  if (A_LDS) {
+    stage_a<NTH>(A, U, B, sA);
+    __syncthreads();
+  }
+  auto aval = [&](uint32_t u, uint32_t b) -> float2 {
+    return (u < U && b < B) ? Asrc[u + (uint64_t)lda * b] : make_float2(0.f, 0.f);
+  };
+  // Gram of tile (mi, mj) over antenna chunk ch: this lane's 4 entries.
+  auto gram_chunk = [&](int mi, int mj, uint32_t ch, float2 (&out)[4]) {
     const uint32_t ui = 16 * mi + i16, uj = 16 * mj + i16;
+    const uint32_t s0 = ch * kGramChunk;
+    const uint32_t s1 = min(s0 + kGramChunk, ks);
     v4f gr1 = {0, 0, 0, 0}, gr2 = {0, 0, 0, 0}, gi1 = {0, 0, 0, 0}, gi2 = {0, 0, 0, 0};
 #pragma unroll 8
-    for (uint32_t s = 0; s < ks; ++s) {
+    for (uint32_t s = s0; s < s1; ++s) {
       const uint32_t b = 4 * s + q4;
-      const bool bok = b < B;
-      const float2 ai = (ui < U && bok) ? A[ui + (uint64_t)U * b] : make_float2(0.f, 0.f);
-      const float2 aj = (mi == mj) ? ai
-                        : ((uj < U && bok) ? A[uj + (uint64_t)U * b] : make_float2(0.f, 0.f));
+      const float2 ai = aval(ui, b);
+      const float2 aj = (mi == mj) ? ai : aval(uj, b);
       gr1 = mfma4(ai.x, aj.x, gr1);
       gr2 = mfma4(ai.y, aj.y, gr2);
       gi1 = mfma4(ai.y, aj.x, gi1);
       gi2 = mfma4(-ai.x, aj.y, gi2);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = 16 * mi + 4 * q4 + r, col = 16 * mj + i16;
-      const float2 v = make_float2(gr1[r] + gr2[r], gi1[r] + gi2[r]);
-      sG[row * LDG + col] = v;
-      if (mi != mj) sG[col * LDG + row] = make_float2(v.x, -v.y);
+    for (int r = 0; r < 4; ++r) out[r] = make_float2(gr1[r] + gr2[r], gi1[r] + gi2[r]);
+  };
+  auto put_tile = [&](int mi, int mj, int l, int r, float2 v) {
+    const int row = 16 * mi + 4 * (l >> 4) + r, col = 16 * mj + (l & 15);
+    sG[row * LDG + col] = v;
+    if (mi != mj) sG[col * LDG + row] = make_float2(v.x, -v.y);
+  };
+
+  // ---- 1. Gram ---------------------------------------------------------------
+  if (!Plan::SPLIT) {
+    for (int tile = wave; tile < Plan::NT; tile += NW) {
+      int mi, mj;
+      tile_of(tile, mi, mj);
+      float2 sum[4], c[4];
+      gram_chunk(mi, mj, 0, sum);
+      for (uint32_t ch = 1; ch < nch; ++ch) {
+        gram_chunk(mi, mj, ch, c);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[r] = make_float2(sum[r].x + c[r].x, sum[r].y + c[r].y);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) put_tile(mi, mj, lane, r, sum[r]);
+    }
+  } else {
+    for (uint32_t item = wave; item < Plan::NT * nch; item += NW) {
+      const int tile = item / nch;
+      int mi, mj;
+      tile_of(tile, mi, mj);
+      float2 c[4];
+      gram_chunk(mi, mj, item % nch, c);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[item * 256 + r * 64 + lane] = c[r];
+    }
+    __syncthreads();
+    for (int e = tid; e < Plan::NT * 256; e += NTH) {
+      const int tile = e / 256, r = (e % 256) / 64, l = e % 64;
+      int mi, mj;
+      tile_of(tile, mi, mj);
+      float2 v = part[(tile * nch) * 256 + r * 64 + l];
+      for (uint32_t ch = 1; ch < nch; ++ch) {
+        const float2 q = part[(tile * nch + ch) * 256 + r * 64 + l];
+        v = make_float2(v.x + q.x, v.y + q.y);
+      }
+      put_tile(mi, mj, l, r, v);
     }
   }
   __syncthreads();
@@ -91,46 +223,66 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
   }
   __syncthreads();
 
-  // ---- 2. Gauss-Jordan inverse in place ----------------------------------------
-  int fail = 0;
+  // ---- 2. sweeps: sG <- -G^-1 --------------------------------------------------
   const float tol = kPivotTol * (float)U;
-  for (uint32_t k = 0; k < U; ++k) {
-    const float piv = sG[k * LDG + k].x;
-    if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
-      fail = (int)k + 1;  // uniform: every thread read the same value
-      break;
-    }
-    const float ip = 1.0f / piv;
-    float2 nv[EPT];
+  int fail = 0;
+  float2* res = sG;  // where -G^-1 ends up
+  if constexpr (UT == 1) {
+    // One wave, in registers: lane holds column c = lane%16 of rows
+    // i = lane/16 + 4j (j = 0..3).
+    if (wave == 0) {
+      const int c = i16;
+      float2 a[4];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int e = tid + j * NTH;
-      if (e < UP * UP) {
-        const int i = e / UP, c = e % UP;
-        const float2 aik = sG[i * LDG + k], akj = sG[k * LDG + c];
-        float2 v;
-        if (i == (int)k && c == (int)k) {
-          v = make_float2(ip, 0.f);
-        } else if (i == (int)k) {
-          v = make_float2(akj.x * ip, akj.y * ip);
-        } else if (c == (int)k) {
-          v = make_float2(-aik.x * ip, -aik.y * ip);
-        } else {
-          const float2 aij = sG[i * LDG + c];
-          const float pr = aik.x * akj.x - aik.y * akj.y;
-          const float pi = aik.x * akj.y + aik.y * akj.x;
-          v = make_float2(aij.x - pr * ip, aij.y - pi * ip);
+      for (int j = 0; j < 4; ++j) a[j] = sG[(q4 + 4 * j) * LDG + c];
+      for (int k = 0; k < (int)U; ++k) {
+        const int jk = k >> 2;
+        const float2 rowreg = jk == 0 ? a[0] : jk == 1 ? a[1] : jk == 2 ? a[2] : a[3];
+        const float2 akc = shfl2(rowreg, (k & 3) * 16 + c);   // a[k][c]
+        const float piv = __shfl(rowreg.x, (k & 3) * 16 + k, 64);
+        if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
+          fail = k + 1;
+          break;
         }
-        nv[j] = v;
-      }
-    }
-    __syncthreads();
+        const float ip = 1.0f / piv;
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int e = tid + j * NTH;
-      if (e < UP * UP) sG[(e / UP) * LDG + e % UP] = nv[j];
+        for (int j = 0; j < 4; ++j) {
+          const float2 aik = shfl2(a[j], q4 * 16 + k);       // a[i][k], i = q4 + 4j
+          a[j] = sweep_entry(a[j], aik, akc, q4 + 4 * j, c, k, ip);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sG[(q4 + 4 * j) * LDG + c] = a[j];
+      if (lane == 0) sh.diag[0] = __int_as_float(fail);
     }
     __syncthreads();
+    fail = __float_as_int(sh.diag[0]);
+  } else {
+    constexpr int EPT = (UP * UP + NTH - 1) / NTH;
+    float2* cur = sh.G[0];
+    float2* nxt = sh.G[1];
+    for (uint32_t k = 0; k < U; ++k) {
+      const float piv = cur[k * LDG + k].x;
+      if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
+        fail = (int)k + 1;  // uniform: every thread read the same value
+        break;
+      }
+      const float ip = 1.0f / piv;
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const int e = tid + j * NTH;
+        if (e < UP * UP) {
+          const int i = e / UP, c = e % UP;
+          nxt[i * LDG + c] = sweep_entry(cur[i * LDG + c], cur[i * LDG + k],
+                                         cur[k * LDG + c], i, c, (int)k, ip);
+        }
+      }
+      __syncthreads();
+      float2* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    res = cur;
   }
   if (fail) {
     for (uint32_t e = tid; e < U * B; e += NTH) {
@@ -141,28 +293,26 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
     return fail;
   }
 
-  // ---- 3. W = G^-1 A -----------------------------------------------------------
+  // ---- 3. W = G^-1 A = -(res) A -------------------------------------------------
   constexpr int KSU = UP / 4;  // K steps over users
   const uint32_t nb_tiles = (B + 15) / 16;
   for (uint32_t nb = wave; nb < nb_tiles; nb += NW) {
     const uint32_t b = 16 * nb + i16;
     float2 av[KSU];
 #pragma unroll
-    for (int s = 0; s < KSU; ++s) {
-      const uint32_t u = 4 * s + q4;
-      av[s] = (u < U && b < B) ? A[u + (uint64_t)U * b] : make_float2(0.f, 0.f);
-    }
+    for (int s = 0; s < KSU; ++s) av[s] = aval(4 * s + q4, b);
     v4f res_r[UT], res_i[UT];
 #pragma unroll
     for (int mi = 0; mi < UT; ++mi) {
       v4f wr1 = {0, 0, 0, 0}, wr2 = {0, 0, 0, 0}, wi1 = {0, 0, 0, 0}, wi2 = {0, 0, 0, 0};
 #pragma unroll
       for (int s = 0; s < KSU; ++s) {
-        const float2 gv = sG[(16 * mi + i16) * LDG + 4 * s + q4];
-        wr1 = mfma4(gv.x, av[s].x, wr1);
-        wr2 = mfma4(-gv.y, av[s].y, wr2);
-        wi1 = mfma4(gv.x, av[s].y, wi1);
-        wi2 = mfma4(gv.y, av[s].x, wi2);
+        const float2 m = res[(16 * mi + i16) * LDG + 4 * s + q4];
+        const float gr = -m.x, gi = -m.y;  // G^-1 = -res
+        wr1 = mfma4(gr, av[s].x, wr1);
+        wr2 = mfma4(-gi, av[s].y, wr2);
+        wi1 = mfma4(gr, av[s].y, wi1);
+        wi2 = mfma4(gi, av[s].x, wi2);
       }
       res_r[mi] = wr1 + wr2;
       res_i[mi] = wi1 + wi2;
