@@ -39,6 +39,12 @@ extern "C" {
 
 /* Numerical form of the ZF / equalize kernels. */
 #define MEGA_FORM_FP32 0   /* plain FP32 arithmetic: the drop-in form */
+/* Products on the BF16 matrix cores with each FP32 operand split into a
+ * hi + lo BF16 pair (hi*hi + hi*lo + lo*hi, FP32 accumulation); FP32 sweep
+ * inverse with one refinement step whose residual is formed in FP64; groups
+ * whose Gram matrix has 1-norm condition number above 1e3 are computed in
+ * FP64 throughout. mega_zf and mega_detect_slots only. */
+#define MEGA_FORM_MIXED 1
 
 typedef void *mega_stream_t; /* hipStream_t */
 typedef struct mega_beamform *mega_beamform_t;
@@ -95,6 +101,16 @@ int mega_zf(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
             const void *csi_in, void *w_out, int32_t *info_dev, int form,
             mega_stream_t stream);
 
+/* Mixed-form ZF (MEGA_FORM_MIXED) emitting the weights as split BF16 words
+ * for mega_equalize_slots with form MEGA_FORM_MIXED: w_split receives
+ * batch * B * U 8-byte words in the CSI layout ({hi, lo} pairs of packed
+ * (re, im) BF16). w_out (optional) receives W in FP32, route_dev (optional)
+ * 1 for groups computed on the FP64 route. No reference counterpart: the
+ * first half of mega_detect_slots with eq.form = MEGA_FORM_MIXED. */
+int mega_zf_split(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
+                  const void *csi_in, void *w_split, void *w_out,
+                  int32_t *info_dev, int32_t *route_dev, mega_stream_t stream);
+
 /* ---- Equalize + demod -------------------------------------------------- */
 
 /* Replaces the raw overload Equalize::equalize(problem_size_row = U,
@@ -119,7 +135,10 @@ int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
  *   y    symbol t of slot s at y + s*y_slot_stride + t*y_sym_stride ([B][N])
  *   llr  at llr + s*llr_slot_stride + t*llr_sym_stride ([U][ue_stride])
  *   xhat optional debug output, complex64 [slots][syms][U][N] (NULL = none).
- * Extends equalize.cu:20-51, which handles one symbol per launch. */
+ * Extends equalize.cu:20-51, which handles one symbol per launch.
+ * With form = MEGA_FORM_MIXED, csi holds the split words of mega_zf_split
+ * (same indexing and slot stride) and the products run on the BF16 matrix
+ * cores with split operands. */
 typedef struct {
   uint32_t num_ue, num_bs, num_sc, sc_group; /* U, B, N, sg */
   uint32_t num_syms, num_slots;
@@ -146,14 +165,23 @@ int mega_equalize_slots(const mega_equalize_desc *desc, mega_stream_t stream);
  * mega_zf(eq.csi -> w_out) followed by mega_equalize_slots(w_out, ...).
  * No single reference counterpart: it fuses Beamform::beamform_uplink
  * (beamform.cu:144-152) with the S Equalize::equalize calls of a slot
- * (process.cc:172-185). */
+ * (process.cc:172-185). With eq.form = MEGA_FORM_MIXED the mixed ZF kernel
+ * writes split BF16 weights into `work` (and FP32 W into w_out if given)
+ * and the mixed equalize kernel reads them, for any shape. */
 typedef struct {
   mega_equalize_desc eq;
   void *w_out;       /* optional: W, same layout and strides as eq.csi */
   int32_t *info_dev; /* optional: device int32[num_slots * G] group status */
+  /* MEGA_FORM_MIXED only: */
+  void *work;         /* device scratch of mega_detect_work_bytes(desc) bytes */
+  int32_t *route_dev; /* optional: device int32[num_slots * G], 1 = FP64 route */
 } mega_detect_desc;
 
 int mega_detect_slots(const mega_detect_desc *desc, mega_stream_t stream);
+
+/* Bytes of device scratch mega_detect_slots needs for desc (0 for
+ * MEGA_FORM_FP32; num_slots * G * B * U * 8 for MEGA_FORM_MIXED). */
+uint64_t mega_detect_work_bytes(const mega_detect_desc *desc);
 
 /* ---- host demodulation ----------------------------------------------------
  * Replaces the host side of demodQPSK / demod16QAM / demod64QAM /

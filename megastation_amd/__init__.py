@@ -29,6 +29,7 @@ MEGA_ERR_BATCH = -3
 MEGA_ERR_HIP = -4
 
 FORM_FP32 = 0
+FORM_MIXED = 1
 
 _lib = None
 
@@ -61,7 +62,8 @@ class EqualizeDesc(ctypes.Structure):
 class DetectDesc(ctypes.Structure):
     """Mirror of ``mega_detect_desc`` (include/megadetect.h)."""
     _fields_ = [("eq", EqualizeDesc), ("w_out", ctypes.c_void_p),
-                ("info_dev", ctypes.c_void_p)]
+                ("info_dev", ctypes.c_void_p), ("work", ctypes.c_void_p),
+                ("route_dev", ctypes.c_void_p)]
 
 
 def declared_symbols() -> list[str]:
@@ -92,10 +94,12 @@ def lib() -> ctypes.CDLL:
         "mega_beamform_downlink": ([P, P, u32, u32, u32, P], i32),
         "mega_beamform_info": ([P, P, u32, P], i32),
         "mega_zf": ([u32, u32, u32, P, P, P, i32, P], i32),
+        "mega_zf_split": ([u32, u32, u32, P, P, P, P, P, P], i32),
         "mega_equalize": ([u64, u64, u64, P, P, P, u64, u64, u8, u64, u64, u64,
                            u64, P], i32),
         "mega_equalize_slots": ([ctypes.POINTER(EqualizeDesc), P], i32),
         "mega_detect_slots": ([ctypes.POINTER(DetectDesc), P], i32),
+        "mega_detect_work_bytes": ([ctypes.POINTER(DetectDesc)], u64),
         "mega_demod_host": ([P, u64, u8, P], i32),
     }
     for name, (args, res) in sig.items():
@@ -191,6 +195,14 @@ def zf(csi_in, w_out, num_bs: int, num_ue: int, batch: int, info=None,
                          _ptr(info), form, _stream(stream)))
 
 
+def zf_split(csi_in, w_split, num_bs: int, num_ue: int, batch: int, w_out=None,
+             info=None, route=None, stream=None):
+    """Mixed-form ZF: split BF16 weight words (8 bytes per weight, CSI layout)
+    for equalize_slots(form=FORM_MIXED, csi=w_split)."""
+    _check(lib().mega_zf_split(num_bs, num_ue, batch, _ptr(csi_in), _ptr(w_split),
+                               _ptr(w_out), _ptr(info), _ptr(route), _stream(stream)))
+
+
 # ---------------------------------------------------------------------------
 # Equalize (src/operators/sgemv/equalize.h:25-74)
 # ---------------------------------------------------------------------------
@@ -244,15 +256,31 @@ def detect_slots(csi, y, llr, *, num_ue, num_bs, num_sc, sc_group, num_syms,
                  num_slots, mod_order, ue_stride, cb_size, cb_stride, zeros,
                  valid_bits, csi_slot_stride, y_sym_stride, y_slot_stride,
                  llr_sym_stride, llr_slot_stride, xhat=None, w_out=None,
-                 info=None, form: int = FORM_FP32, stream=None):
-    """Fused ZF + equalize + demod: csi holds the channel estimates A = H^H."""
+                 info=None, form: int = FORM_FP32, work=None, route=None,
+                 stream=None):
+    """Fused ZF + equalize + demod: csi holds the channel estimates A = H^H.
+    form=FORM_MIXED needs `work` (detect_work_bytes(...) bytes of device
+    memory, e.g. a torch uint8 tensor); `route` (int32, num_slots*G) gets 1
+    for groups computed on the FP64 route."""
     e = EqualizeDesc(num_ue, num_bs, num_sc, sc_group, num_syms, num_slots,
                      mod_order, form, ue_stride, cb_size, cb_stride, zeros,
                      valid_bits, _ptr(csi).value, csi_slot_stride, _ptr(y).value,
                      y_sym_stride, y_slot_stride, _ptr(llr).value, llr_sym_stride,
                      llr_slot_stride, _ptr(xhat).value)
-    d = DetectDesc(e, _ptr(w_out).value, _ptr(info).value)
+    d = DetectDesc(e, _ptr(w_out).value, _ptr(info).value, _ptr(work).value,
+                   _ptr(route).value)
     _check(lib().mega_detect_slots(ctypes.byref(d), _stream(stream)))
+
+
+def detect_work_bytes(*, num_ue, num_bs, num_sc, sc_group, num_slots,
+                      form: int = FORM_FP32) -> int:
+    """Device scratch bytes detect_slots needs (mega_detect_work_bytes)."""
+    e = EqualizeDesc()
+    e.num_ue, e.num_bs, e.num_sc, e.sc_group = num_ue, num_bs, num_sc, sc_group
+    e.num_syms, e.num_slots, e.form = 1, num_slots, form
+    d = DetectDesc()
+    d.eq = e
+    return int(lib().mega_detect_work_bytes(ctypes.byref(d)))
 
 
 def demod_host(xhat, mod_order: int):

@@ -20,7 +20,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmegadetect.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["capi.cpp", "equalize.hip", "zf.hip", "detect.hip"]
+SOURCES = ["capi.cpp", "equalize.hip", "zf.hip", "detect.hip", "zf_mixed.hip",
+           "equalize_mixed.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

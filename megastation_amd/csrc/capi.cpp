@@ -74,8 +74,9 @@ int mega_beamform_destroy(mega_beamform_t bf) {
 static int zf_checked(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
                       const void* in, void* out, int32_t* info, int form,
                       mega_stream_t stream, const char* who) {
-  if (form != MEGA_FORM_FP32)
-    return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": unknown form");
+  if (form != MEGA_FORM_FP32 && form != MEGA_FORM_MIXED)
+    return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": unsupported form " +
+                                          std::to_string(form));
   if (batch == 0) return MEGA_OK;
   if (!in || !out) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
   if (num_ue == 0 || num_bs == 0)
@@ -83,10 +84,31 @@ static int zf_checked(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
   if (num_ue > mega::detail::kMaxUsers)
     return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": num_ue > 64");
   const char* why = nullptr;
-  hipError_t e = mega::detail::launch_zf(
-      num_ue, num_bs, batch, static_cast<const float2*>(in),
-      static_cast<float2*>(out), info, static_cast<hipStream_t>(stream), &why);
+  hipError_t e =
+      form == MEGA_FORM_MIXED
+          ? mega::detail::launch_zf_mixed(num_ue, num_bs, batch, static_cast<const float2*>(in),
+                                          static_cast<float2*>(out), nullptr, info, nullptr,
+                                          static_cast<hipStream_t>(stream), &why)
+          : mega::detail::launch_zf(num_ue, num_bs, batch, static_cast<const float2*>(in),
+                                    static_cast<float2*>(out), info,
+                                    static_cast<hipStream_t>(stream), &why);
   if (e != hipSuccess) return hip_fail(e, who, why);
+  return MEGA_OK;
+}
+
+int mega_zf_split(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
+                  const void* csi_in, void* w_split, void* w_out,
+                  int32_t* info_dev, int32_t* route_dev, mega_stream_t stream) {
+  if (batch == 0) return MEGA_OK;
+  if (!csi_in || !w_split) return fail(MEGA_ERR_INVALID_ARG, "mega_zf_split: NULL buffer");
+  if (num_ue == 0 || num_bs == 0) return fail(MEGA_ERR_INVALID_ARG, "mega_zf_split: zero size");
+  if (num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_zf_split: num_ue > 64");
+  const char* why = nullptr;
+  hipError_t e = mega::detail::launch_zf_mixed(
+      num_ue, num_bs, batch, static_cast<const float2*>(csi_in), static_cast<float2*>(w_out),
+      w_split, info_dev, route_dev, static_cast<hipStream_t>(stream), &why);
+  if (e != hipSuccess) return hip_fail(e, "mega_zf_split", why);
   return MEGA_OK;
 }
 
@@ -147,13 +169,15 @@ int mega_zf(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
 }
 
 static int eq_args_checked(const mega_equalize_desc* d, mega::detail::EqualizeArgs& a,
-                           bool& empty, const char* who) {
+                           bool& empty, const char* who, bool allow_mixed = false) {
   empty = false;
   if (!d) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": desc is NULL");
   if (!valid_q(d->mod_order))
     return fail(MEGA_ERR_UNSUPPORTED,
                 "unsupported modulation order " + std::to_string(d->mod_order));
-  if (d->form != MEGA_FORM_FP32) return fail(MEGA_ERR_UNSUPPORTED, "unknown form");
+  if (d->form != MEGA_FORM_FP32 && !(allow_mixed && d->form == MEGA_FORM_MIXED))
+    return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": unsupported form " +
+                                          std::to_string(d->form));
   if (d->num_ue == 0 || d->num_bs == 0 || d->num_sc == 0 || d->num_syms == 0 ||
       d->num_slots == 0) {
     empty = true;
@@ -195,10 +219,13 @@ static int eq_args_checked(const mega_equalize_desc* d, mega::detail::EqualizeAr
 int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
   mega::detail::EqualizeArgs a;
   bool empty;
-  int r = eq_args_checked(d, a, empty, "mega_equalize_slots");
+  int r = eq_args_checked(d, a, empty, "mega_equalize_slots", true);
   if (r != MEGA_OK || empty) return r;
   const char* why = nullptr;
-  hipError_t e = mega::detail::launch_equalize(a, static_cast<hipStream_t>(stream), &why);
+  hipError_t e = d->form == MEGA_FORM_MIXED
+                     ? mega::detail::launch_equalize_mixed(a, d->csi,
+                                                           static_cast<hipStream_t>(stream), &why)
+                     : mega::detail::launch_equalize(a, static_cast<hipStream_t>(stream), &why);
   if (e != hipSuccess) return hip_fail(e, "mega_equalize", why);
   return MEGA_OK;
 }
@@ -207,8 +234,17 @@ int mega_detect_slots(const mega_detect_desc* d, mega_stream_t stream) {
   if (!d) return fail(MEGA_ERR_INVALID_ARG, "mega_detect_slots: desc is NULL");
   mega::detail::EqualizeArgs a;
   bool empty;
-  int r = eq_args_checked(&d->eq, a, empty, "mega_detect_slots");
+  int r = eq_args_checked(&d->eq, a, empty, "mega_detect_slots", true);
   if (r != MEGA_OK || empty) return r;
+  if (d->eq.form == MEGA_FORM_MIXED) {
+    if (!d->work) return fail(MEGA_ERR_INVALID_ARG, "mega_detect_slots: the mixed form needs work");
+    const char* why = nullptr;
+    hipError_t e = mega::detail::launch_detect_mixed(
+        a, static_cast<float2*>(d->w_out), d->work, d->info_dev, d->route_dev,
+        static_cast<hipStream_t>(stream), &why);
+    if (e != hipSuccess) return hip_fail(e, "mega_detect_slots", why);
+    return MEGA_OK;
+  }
   if (!d->w_out && !mega::detail::detect_fused_supported(a.U, a.B))
     return fail(MEGA_ERR_INVALID_ARG,
                 "mega_detect_slots: U > 16 or B > 256 needs w_out");
@@ -218,6 +254,12 @@ int mega_detect_slots(const mega_detect_desc* d, mega_stream_t stream) {
                                              static_cast<hipStream_t>(stream), &why);
   if (e != hipSuccess) return hip_fail(e, "mega_detect_slots", why);
   return MEGA_OK;
+}
+
+uint64_t mega_detect_work_bytes(const mega_detect_desc* d) {
+  if (!d || d->eq.form != MEGA_FORM_MIXED || d->eq.sc_group == 0) return 0;
+  const uint64_t G = (d->eq.num_sc + d->eq.sc_group - 1) / d->eq.sc_group;
+  return (uint64_t)d->eq.num_slots * G * d->eq.num_bs * d->eq.num_ue * 8;
 }
 
 int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,

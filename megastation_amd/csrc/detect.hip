@@ -114,7 +114,14 @@ hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
       *why = "shape outside the fused kernel (U > 16 or B > 256) needs w_out";
       return hipErrorInvalidValue;
     }
-    hipError_t e = launch_zf(a.U, a.B, a.slots * a.G, a.csi, w_out, info, st, why);
+    hipError_t e = hipSuccess;
+    if (a.csi_slot == (uint64_t)a.G * a.U * a.B || a.slots == 1) {
+      e = launch_zf(a.U, a.B, a.slots * a.G, a.csi, w_out, info, st, why);
+    } else {  // padded slot stride (w_out has the same layout): one launch per slot
+      for (uint32_t s = 0; s < a.slots && e == hipSuccess; ++s)
+        e = launch_zf(a.U, a.B, a.G, a.csi + s * a.csi_slot, w_out + s * a.csi_slot,
+                      info ? info + (uint64_t)s * a.G : nullptr, st, why);
+    }
     if (e != hipSuccess) return e;
     EqualizeArgs b = a;
     b.csi = w_out;
@@ -153,6 +160,33 @@ hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
       *why = "no fused configuration";
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_detect_mixed(const EqualizeArgs& a, float2* w_out, void* work,
+                               int32_t* info, int32_t* route, hipStream_t st,
+                               const char** why) {
+  if (a.U == 0 || a.B == 0 || a.N == 0 || a.sg == 0 || a.S == 0 || a.slots == 0)
+    return hipSuccess;
+  if (!work) {
+    *why = "the mixed form needs a work buffer";
+    return hipErrorInvalidValue;
+  }
+  const uint64_t per_slot = (uint64_t)a.G * a.U * a.B;  // weights per slot (dense work)
+  hipError_t e = hipSuccess;
+  if (a.csi_slot == per_slot || a.slots == 1) {
+    e = launch_zf_mixed(a.U, a.B, a.slots * a.G, a.csi, w_out, work, info, route, st, why);
+  } else {
+    for (uint32_t s = 0; s < a.slots && e == hipSuccess; ++s)
+      e = launch_zf_mixed(a.U, a.B, a.G, a.csi + s * a.csi_slot,
+                          w_out ? w_out + s * a.csi_slot : nullptr,
+                          static_cast<uint2*>(work) + s * per_slot,
+                          info ? info + (uint64_t)s * a.G : nullptr,
+                          route ? route + (uint64_t)s * a.G : nullptr, st, why);
+  }
+  if (e != hipSuccess) return e;
+  EqualizeArgs b = a;
+  b.csi_slot = per_slot;
+  return launch_equalize_mixed(b, work, st, why);
 }
 
 }  // namespace detail

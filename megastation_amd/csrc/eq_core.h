@@ -91,6 +91,47 @@ __device__ __forceinline__ void store_llrs(const EqParams& p, _Float16* row,
   }
 }
 
+// Output side of one column (lane%16) of column tile t of group g: which
+// (symbol, subcarrier) it is, where its LLRs go, and the x_hat row.
+struct EqCol {
+  bool valid;
+  LlrSlot ls;
+  _Float16* rows;  // user 0's LLR row of this symbol
+  float2* xrow;    // user 0's x_hat of this (symbol, subcarrier), or null
+};
+
+template <int Q>
+__device__ __forceinline__ EqCol eq_col(const EqParams& p, uint32_t slot, uint32_t g,
+                                        uint32_t t) {
+  const uint32_t k0 = g * p.sg;
+  const uint32_t nk = min(p.sg, p.N - k0);
+  const uint32_t n = 16 * t + (threadIdx.x & 15);
+  EqCol c;
+  c.valid = n < nk * p.S;
+  const uint32_t sym = c.valid ? n / nk : 0;
+  const uint32_t k = k0 + (c.valid ? n - sym * nk : 0);
+  c.ls = llr_slot<Q>(p, k);
+  c.rows = p.llr + slot * p.llr_slot + sym * p.llr_sym;
+  c.xrow = p.xhat ? p.xhat + ((uint64_t)slot * p.S + sym) * p.U * p.N + k : nullptr;
+  return c;
+}
+
+// Demod and store the two equalized symbols of users u0, u0+1 held in an
+// MFMA result (d = re u0, im u0, re u0+1, im u0+1).
+template <int Q>
+__device__ __forceinline__ void eq_store(const EqParams& p, const EqCol& c,
+                                         uint32_t u0, v4f d) {
+  if (!c.valid) return;
+  if (u0 < p.U) {
+    store_llrs<Q>(p, c.rows + u0 * p.ue_stride, c.ls, d[0], d[1]);
+    if (c.xrow) c.xrow[(uint64_t)u0 * p.N] = make_float2(d[0], d[1]);
+  }
+  if (u0 + 1 < p.U) {
+    store_llrs<Q>(p, c.rows + (u0 + 1) * p.ue_stride, c.ls, d[2], d[3]);
+    if (c.xrow) c.xrow[(uint64_t)(u0 + 1) * p.N] = make_float2(d[2], d[3]);
+  }
+}
+
 // Maps the hardware workgroup id to a logical work index so that consecutive
 // logical indices share an XCD (ids are dealt to the 8 XCDs round-robin).
 __device__ __forceinline__ uint32_t xcd_linear(uint32_t w, uint32_t T) {
@@ -184,13 +225,8 @@ __device__ __forceinline__ void eq_tiles(const EqParams& p, const WSrc& ws,
                                          uint32_t t1, float* sRed,
                                          float2 (&ynext)[KS]) {
   const int lane = threadIdx.x & 63;
-  const int i16 = lane & 15, q4 = lane >> 4;
+  const int q4 = lane >> 4;
   const int nw = blockDim.x >> 6;
-  const uint32_t U = p.U;
-  const uint32_t k0 = g * p.sg;
-  const uint32_t nk = min(p.sg, p.N - k0);
-  const uint32_t ncols = nk * p.S;
-  _Float16* L = p.llr + slot * p.llr_slot;
   int buf = 0;
 
   for (uint32_t t = t0; t < t1; ++t) {
@@ -203,11 +239,6 @@ __device__ __forceinline__ void eq_tiles(const EqParams& p, const WSrc& ws,
     }
     if (PREFETCH && t + 1 < t1)
       eq_load_tile<KS>(p, slot, g, wk, t + 1, ynext);  // in flight during the MFMAs
-
-    const uint32_t n = 16 * t + i16;
-    const bool valid_n = n < ncols;
-    const uint32_t sym = valid_n ? n / nk : 0;
-    const uint32_t kk = valid_n ? n - sym * nk : 0;
 
     v4f acc1[MT], acc2[MT];
 #pragma unroll
@@ -226,22 +257,8 @@ __device__ __forceinline__ void eq_tiles(const EqParams& p, const WSrc& ws,
       }
     }
 
-    const uint32_t k = k0 + kk;
-    const LlrSlot ls = llr_slot<Q>(p, k);
-    _Float16* rows = L + sym * p.llr_sym;
-    float2* xrow = p.xhat ? p.xhat + ((uint64_t)slot * p.S + sym) * U * p.N + k : nullptr;
-    auto epilogue = [&](int m, v4f d) {
-      if (!valid_n) return;
-      const uint32_t u0 = 8 * (mg * MT + m) + 2 * q4;
-      if (u0 < U) {
-        store_llrs<Q>(p, rows + u0 * p.ue_stride, ls, d[0], d[1]);
-        if (xrow) xrow[(uint64_t)u0 * p.N] = make_float2(d[0], d[1]);
-      }
-      if (u0 + 1 < U) {
-        store_llrs<Q>(p, rows + (u0 + 1) * p.ue_stride, ls, d[2], d[3]);
-        if (xrow) xrow[(uint64_t)(u0 + 1) * p.N] = make_float2(d[2], d[3]);
-      }
-    };
+    const EqCol col = eq_col<Q>(p, slot, g, t);
+    auto epilogue = [&](int m, v4f d) { eq_store<Q>(p, col, 8 * (mg * MT + m) + 2 * q4, d); };
 
     if (p.nwk == 1) {
 #pragma unroll
@@ -270,6 +287,90 @@ __device__ __forceinline__ void eq_tiles(const EqParams& p, const WSrc& ws,
     }
   }
 }
+
+// One column tile t of group g for users 8*MT*mg ... 8*MT*(mg+1)-1, by one
+// wave over all B antennas, with W streamed from memory instead of held: for
+// groups whose W slice does not fit a wave's registers. The antennas go in
+// chunks of KC steps; y (the HBM stream) is kept RY-1 chunks ahead in a ring
+// of RY register sets, W (re-read by every tile of the group, so mostly L2
+// hits) one chunk ahead. No LDS and no barriers: each wave is independent.
+//
+// Requires B % (4*KC*RY) == 0 (eq_stream_ok). Every load is then in bounds
+// without a mask: columns past the group's end re-read its last column and
+// users past U re-read user U-1 (their results are never stored), and the
+// look-ahead past the last chunk re-reads that chunk. With no branches or
+// selects around the loads the compiler waits for exactly the chunk in use.
+template <int MT, int KC, int RY, int Q>
+__device__ __forceinline__ void eq_unit_stream(const EqParams& p, uint32_t slot,
+                                               uint32_t g, uint32_t t, uint32_t mg) {
+  static_assert(RY % 2 == 0, "the W ring (2) must divide the y ring");
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const bool odd = i16 & 1;
+  const uint32_t U = p.U, B = p.B, N = p.N;
+  const uint32_t k0 = g * p.sg;
+  const uint32_t nk = min(p.sg, N - k0);
+  const uint32_t n = min(16 * t + i16, nk * p.S - 1);
+  const uint32_t sym = n / nk;
+  const float2* ycol = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk) + q4 * N;
+  const float2* W = p.csi + slot * p.csi_slot + (uint64_t)g * U * B;
+  const float2* wrow[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+    wrow[m] = W + min(8 * (mg * MT + m) + (i16 >> 1), U - 1) + (uint64_t)q4 * U;
+  const uint32_t nch = B / (4 * KC);
+
+  auto load_y = [&](uint32_t c, float2 (&dst)[KC]) {
+    c = min(c, nch - 1);
+#pragma unroll
+    for (int s = 0; s < KC; ++s) dst[s] = ycol[(uint64_t)(4 * (c * KC + s)) * N];
+  };
+  auto load_w = [&](uint32_t c, float2 (&dst)[MT][KC]) {
+    c = min(c, nch - 1);
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int s = 0; s < KC; ++s) dst[m][s] = wrow[m][(uint64_t)(4 * (c * KC + s)) * U];
+  };
+
+  float2 yr[RY][KC];
+  float2 wr[2][MT][KC];
+#pragma unroll
+  for (int r = 0; r + 1 < RY; ++r) load_y(r, yr[r]);
+  load_w(0, wr[0]);
+
+  v4f acc1[MT], acc2[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    acc1[m] = v4f{0.f, 0.f, 0.f, 0.f};
+    acc2[m] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  for (uint32_t c0 = 0; c0 < nch; c0 += RY) {
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const uint32_t c = c0 + r;
+      load_y(c + RY - 1, yr[(r + RY - 1) % RY]);
+      load_w(c + 1, wr[(r + 1) % 2]);
+      // keep the look-ahead loads ahead of this chunk's MFMAs (the scheduler
+      // would otherwise sink them next to their uses)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int s = 0; s < KC; ++s)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const float2 v = wr[r % 2][m][s];
+          acc1[m] = mfma4(odd ? v.y : v.x, yr[r][s].x, acc1[m]);
+          acc2[m] = mfma4(odd ? v.x : -v.y, yr[r][s].y, acc2[m]);
+        }
+    }
+  }
+  const EqCol col = eq_col<Q>(p, slot, g, t);
+#pragma unroll
+  for (int m = 0; m < MT; ++m) eq_store<Q>(p, col, 8 * (mg * MT + m) + 2 * q4, acc1[m] + acc2[m]);
+}
+
+// Shapes eq_unit_stream<MT, KC, RY, Q> takes.
+inline bool eq_stream_ok(uint32_t B, int KC, int RY) { return B % (4 * KC * RY) == 0; }
 
 // Fills the launch-independent fields of EqParams from the C-ABI arguments
 // (equalize.hip).

@@ -49,6 +49,43 @@ __global__ __launch_bounds__(64 * max_waves(MT, KS)) void equalize_mfma(const Eq
   eq_tiles<MT, KS, Q>(p, ws, slot, g, mg, wk, t0, t1, sRed, ynext);
 }
 
+// Streamed-W form: one wave per (slot, group, column tile, user group), four
+// independent waves per workgroup, units numbered so that a group's tiles
+// share an XCD (and its L2 copy of W_g).
+template <int MT, int KC, int RY, int Q>
+__global__ __launch_bounds__(256) void equalize_stream(const EqParams p, uint32_t units) {
+  const uint32_t unit = xcd_linear(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (unit >= units) return;
+  const uint32_t mg = unit % p.nmg;
+  const uint32_t t = (unit / p.nmg) % p.col_tiles;
+  const uint32_t rest = unit / p.nmg / p.col_tiles;
+  const uint32_t g = rest % p.G, slot = rest / p.G;
+  if (16 * t >= min(p.sg, p.N - g * p.sg) * p.S) return;  // past a partial group
+  eq_unit_stream<MT, KC, RY, Q>(p, slot, g, t, mg);
+}
+
+template <int MT, int KC, int RY>
+hipError_t launch_stream_q(const EqParams& p, uint32_t units, uint32_t Q, hipStream_t st) {
+  const dim3 grid((units + 3) / 4), block(256);
+  switch (Q) {
+    case 2: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 2>), grid, block, 0, st, p, units); break;
+    case 4: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 4>), grid, block, 0, st, p, units); break;
+    case 6: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 6>), grid, block, 0, st, p, units); break;
+    case 8: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 8>), grid, block, 0, st, p, units); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Chunk shape (KC antenna steps, ring of RY y chunks): 2x4 by default,
+// MEGA_EQ_SKC=4 selects 4x2 (tuning sweeps). B must be a multiple of 32.
+template <int MT>
+hipError_t launch_stream(const EqParams& p, uint32_t units, uint32_t Q, int kc,
+                         hipStream_t st) {
+  if (kc == 4) return launch_stream_q<MT, 4, 2>(p, units, Q, st);
+  return launch_stream_q<MT, 2, 4>(p, units, Q, st);
+}
+
 template <int MT, int KS, int Q>
 hipError_t launch_one(const EqParams& p, dim3 grid, dim3 block, size_t lds,
                       hipStream_t st) {
@@ -127,6 +164,29 @@ hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t st,
   if (e != hipSuccess) return e;
   // Tile plan; MEGA_EQ_KS=8|16 overrides the antenna step count for sweeps.
   const EqPlan pl = plan_equalize(a.U, a.B, (uint32_t)env_int("MEGA_EQ_KS"));
+  // W held in registers while one wave covers all antennas; otherwise W is
+  // streamed (MEGA_EQ_MODE=1 forces streaming, 2 forbids it).
+  const int mode = env_int("MEGA_EQ_MODE");
+  const int kc = env_int("MEGA_EQ_SKC") == 4 ? 4 : 2;
+  if (mode != 2 && (mode == 1 || pl.nwk > 1) && eq_stream_ok(a.B, kc, kc == 4 ? 2 : 4)) {
+    uint32_t MT = a.U <= 8 ? 1 : a.U <= 16 ? 2 : 4;
+    const int smt = env_int("MEGA_EQ_SMT");  // users per wave / 8 (sweeps)
+    if (smt == 1 || smt == 2 || smt == 4) MT = (uint32_t)smt;
+    p.nmg = (a.U + 8 * MT - 1) / (8 * MT);
+    p.nwk = 1;
+    p.chunks = 1;
+    p.tiles_per_wg = p.col_tiles;
+    const uint64_t units = (uint64_t)a.slots * a.G * p.col_tiles * p.nmg;
+    if (units > 0xffffffffull - 3) {
+      *why = "grid too large";
+      return hipErrorInvalidValue;
+    }
+    switch (MT) {
+      case 1: return launch_stream<1>(p, (uint32_t)units, a.Q, kc, st);
+      case 2: return launch_stream<2>(p, (uint32_t)units, a.Q, kc, st);
+      default: return launch_stream<4>(p, (uint32_t)units, a.Q, kc, st);
+    }
+  }
   if (pl.nwk > (uint32_t)max_waves(pl.MT, pl.KS)) {
     *why = "too many waves per workgroup for the register budget (U*B too large)";
     return hipErrorInvalidValue;

@@ -35,6 +35,22 @@ hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
                          hipStream_t stream, const char** why);
 bool detect_fused_supported(uint32_t U, uint32_t B);
 
+// Mixed form (split-BF16 matrix-core products; zf_mixed.hip): W as FP32
+// (w_out, optional) and as split BF16 words (wp, optional, 8 bytes per
+// weight, CSI indexing); route[g] = 1 where the group took the FP64 route.
+hipError_t launch_zf_mixed(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                           float2* w_out, void* wp, int32_t* info, int32_t* route,
+                           hipStream_t stream, const char** why);
+// Equalize + demod from the split words (a.csi unused; wp with CSI indexing
+// and slot stride a.csi_slot).
+hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp,
+                                 hipStream_t stream, const char** why);
+// Mixed-form detection: zf_mixed into `work` (slots*G*B*U*8 bytes), then the
+// mixed equalize.
+hipError_t launch_detect_mixed(const EqualizeArgs& a, float2* w_out, void* work,
+                               int32_t* info, int32_t* route, hipStream_t stream,
+                               const char** why);
+
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,
                                hipStream_t stream);

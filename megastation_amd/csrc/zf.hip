@@ -24,13 +24,15 @@ __global__ __launch_bounds__(zf_threads<UT>()) void zf_mfma(
 }
 
 // Static LDS of ZfShared<UT> plus the dynamic part must stay within the
-// 160 KB a workgroup may use; A is staged only when that holds.
+// 160 KB a workgroup may use; A is staged (up to 16 users) only when that
+// holds. Wider groups read A from global memory so that several groups fit
+// a CU at once.
 template <int UT>
 hipError_t launch_zf_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
                         float2* out, int32_t* info, hipStream_t st) {
   constexpr int NTH = zf_threads<UT>();
   const size_t with_a = zf_dyn_lds<UT, NTH>(U, B, true);
-  const bool a_lds = sizeof(ZfShared<UT>) + with_a <= 120 * 1024;
+  const bool a_lds = UT == 1 && sizeof(ZfShared<UT>) + with_a <= 120 * 1024;
   const size_t lds = a_lds ? with_a : zf_dyn_lds<UT, NTH>(U, B, false);
   const void* fn = a_lds ? reinterpret_cast<const void*>(&zf_mfma<UT, true>)
                          : reinterpret_cast<const void*>(&zf_mfma<UT, false>);

@@ -5,8 +5,10 @@
 // (Gram), a batched potrf (lower Cholesky) and G*B single-right-hand-side
 // potrs solves through cuBLAS/cuSOLVER. Here the steps are fused per group,
 // both products on the FP32 matrix cores (v_mfma_f32_16x16x4_f32):
-//   0. A is staged into LDS once (rows of U+1 complex, all loads in flight)
-//      when it fits; both products then read it from there.
+//   0. For up to 16 users A is staged into LDS once (rows of U+1 complex, all
+//      loads in flight) and both products read it from there. Larger groups
+//      read A from global memory (L2 for the second product), which keeps the
+//      workgroup's LDS small enough for several groups per CU.
 //   1. Gram G = A A^H, users padded to 16-row tiles, K = 4 antennas per step:
 //        Re G += Re A Re A^T + Im A Im A^T,  Im G += Im A Re A^T - Re A Im A^T.
 //      For the 16x16x4 instruction the A operand of lane l is X[l%16][l/16]
@@ -22,8 +24,9 @@
 //      which after all pivots leaves -G^-1 (no pivoting: G is Hermitian
 //      positive definite and the pivots are exactly the squared Cholesky
 //      diagonals). For up to 16 users one wave does it in registers with
-//      cross-lane shuffles; larger groups ping-pong between two LDS copies
-//      (one barrier per pivot). A pivot not above U * 2^-23 * G_kk marks the
+//      cross-lane shuffles; for larger groups every thread keeps its entries
+//      in registers and the pivot row and column go through LDS (one barrier
+//      per pivot). A pivot not above U * 2^-23 * G_kk marks the
 //      group numerically singular: status k+1 and W = 0 (the reference never
 //      reads cuSOLVER's status and would emit Inf/NaN).
 //   3. W = G^-1 A = -(sweep result) A, M = users, N = 16 antennas, K = 4 users
@@ -42,8 +45,9 @@ template <int UT>
 struct ZfShared {
   static constexpr int UP = 16 * UT;
   static constexpr int LDG = UP + 1;  // padded row (complex elements)
-  static constexpr int NBUF = UT == 1 ? 1 : 2;
-  float2 G[NBUF][UP * LDG];
+  static constexpr int NPUB = UT == 1 ? 1 : 2;  // multi-wave sweep: pivot row/column
+  float2 G[UP * LDG];
+  float2 prow[NPUB][UP], pcol[NPUB][UP];
   float diag[UP];
 };
 
@@ -124,6 +128,105 @@ __device__ __forceinline__ float2 sweep_entry(float2 aic, float2 aik, float2 akc
   return make_float2(base.x - p.x * ip, base.y - p.y * ip);
 }
 
+// Overwrites the Gram matrix in sh.G (Hermitian, users padded to UP with an
+// identity block) with -G^-1 by U pivots of the sweep operator; returns 0 or
+// k+1 for the first pivot k at or below U * 2^-23 * G_kk (then sh.G holds a
+// partial result). All NTH threads of the workgroup call it after a barrier
+// that follows the writes of sh.G; it ends with a barrier.
+template <int UT, int NTH>
+__device__ __forceinline__ int zf_sweep(ZfShared<UT>& sh, uint32_t U) {
+  constexpr int UP = ZfShared<UT>::UP;
+  constexpr int LDG = ZfShared<UT>::LDG;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  float2* sG = sh.G;
+  for (int i = tid; i < UP; i += NTH) {
+    if (i >= (int)U) sG[i * LDG + i] = make_float2(1.f, 0.f);
+    sh.diag[i] = sG[i * LDG + i].x;
+  }
+  __syncthreads();
+
+  const float tol = kPivotTol * (float)U;
+  int fail = 0;
+  if constexpr (UT == 1) {
+    // One wave, in registers: lane holds column c = lane%16 of rows
+    // i = lane/16 + 4j (j = 0..3).
+    if (wave == 0) {
+      const int c = i16;
+      float2 a[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[j] = sG[(q4 + 4 * j) * LDG + c];
+      for (int k = 0; k < (int)U; ++k) {
+        const int jk = k >> 2;
+        const float2 rowreg = jk == 0 ? a[0] : jk == 1 ? a[1] : jk == 2 ? a[2] : a[3];
+        const float2 akc = shfl2(rowreg, (k & 3) * 16 + c);   // a[k][c]
+        const float piv = __shfl(rowreg.x, (k & 3) * 16 + k, 64);
+        if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
+          fail = k + 1;
+          break;
+        }
+        const float ip = 1.0f / piv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float2 aik = shfl2(a[j], q4 * 16 + k);       // a[i][k], i = q4 + 4j
+          a[j] = sweep_entry(a[j], aik, akc, q4 + 4 * j, c, k, ip);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sG[(q4 + 4 * j) * LDG + c] = a[j];
+      if (lane == 0) sh.diag[0] = __int_as_float(fail);
+    }
+    __syncthreads();
+    fail = __float_as_int(sh.diag[0]);
+  } else {
+    // Each thread keeps EPT entries in registers: column c = tid % UP of rows
+    // i = tid / UP + RS*j. Per pivot only row k and column k are needed from
+    // other threads; their owners publish them (double-buffered, so one
+    // barrier per pivot suffices) after updating them for the previous pivot.
+    static_assert((UP * UP) % NTH == 0 && NTH % UP == 0, "sweep layout");
+    constexpr int EPT = UP * UP / NTH;
+    constexpr int RS = NTH / UP;
+    const int c = tid % UP, r0 = tid / UP;
+    float2 a[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) a[j] = sG[(r0 + RS * j) * LDG + c];
+    auto publish = [&](int k, int buf) {
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const int i = r0 + RS * j;
+        if (i == k) sh.prow[buf][c] = a[j];
+        if (c == k) sh.pcol[buf][i] = a[j];
+      }
+    };
+    publish(0, 0);
+    __syncthreads();
+    for (uint32_t k = 0; k < U; ++k) {
+      const int buf = k & 1;
+      const float piv = sh.prow[buf][k].x;
+      if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
+        fail = (int)k + 1;  // uniform: every thread read the same value
+        break;
+      }
+      const float ip = 1.0f / piv;
+      const float2 akc = sh.prow[buf][c];
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const int i = r0 + RS * j;
+        a[j] = sweep_entry(a[j], sh.pcol[buf][i], akc, i, c, (int)k, ip);
+      }
+      if (k + 1 < U) publish((int)k + 1, buf ^ 1);
+      __syncthreads();
+    }
+    if (!fail) {
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) sG[(r0 + RS * j) * LDG + c] = a[j];
+    }
+    __syncthreads();
+  }
+  return fail;
+}
+
 // Returns the group status (0 = ok, k+1 = pivot k failed), identical in every
 // thread. A is read from global memory (ld = U) or, when A_LDS, first staged
 // into sA (rows of U+1). W is written as W[u + ld_w * b] (ld_w = U for the
@@ -144,7 +247,7 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
   const uint32_t nch = gram_chunks(B);
   const uint32_t lda = A_LDS ? U + 1 : U;
   const float2* Asrc = A_LDS ? sA : A;
-  float2* sG = sh.G[0];
+  float2* sG = sh.G;
 
   if (A_LDS) {
     stage_a<NTH>(A, U, B, sA);
@@ -155,25 +258,50 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
   };
   // Gram of tile (mi, mj) over antenna chunk ch: this lane's 4 entries.
   auto gram_chunk = [&](int mi, int mj, uint32_t ch, float2 (&out)[4]) {
-    const uint32_t ui = 16 * mi + i16, uj = 16 * mj + i16;
     const uint32_t s0 = ch * kGramChunk;
-    const uint32_t s1 = min(s0 + kGramChunk, ks);
     v4f gr1 = {0, 0, 0, 0}, gr2 = {0, 0, 0, 0}, gi1 = {0, 0, 0, 0}, gi2 = {0, 0, 0, 0};
-#pragma unroll 8
-    for (uint32_t s = s0; s < s1; ++s) {
-      const uint32_t b = 4 * s + q4;
-      const float2 ai = aval(ui, b);
-      const float2 aj = (mi == mj) ? ai : aval(uj, b);
+    auto step = [&](float2 ai, float2 aj) {
       gr1 = mfma4(ai.x, aj.x, gr1);
       gr2 = mfma4(ai.y, aj.y, gr2);
       gi1 = mfma4(ai.y, aj.x, gi1);
       gi2 = mfma4(-ai.x, aj.y, gi2);
+    };
+    if constexpr (A_LDS) {
+      const uint32_t ui = 16 * mi + i16, uj = 16 * mj + i16;
+      const uint32_t s1 = min(s0 + kGramChunk, ks);
+#pragma unroll 8
+      for (uint32_t s = s0; s < s1; ++s) {
+        const uint32_t b = 4 * s + q4;
+        const float2 ai = aval(ui, b);
+        step(ai, (mi == mj) ? ai : aval(uj, b));
+      }
+    } else {
+      // From global memory: all loads of the chunk issued up front, with
+      // clamped (always valid) addresses and no branches. Rows past U carry
+      // clamped data and are cleared in put_tile; antennas past B are
+      // multiplied by 0 (the steps past the last antenna then add exact
+      // zeros, so the sum equals the one over the valid steps only).
+      const uint32_t ui = min(16u * mi + i16, U - 1), uj = min(16u * mj + i16, U - 1);
+      float2 ai[kGramChunk], aj[kGramChunk];
+#pragma unroll
+      for (uint32_t s = 0; s < kGramChunk; ++s) {
+        const uint32_t b = min(4 * (s0 + s) + q4, B - 1);
+        ai[s] = A[ui + (uint64_t)U * b];
+        aj[s] = A[uj + (uint64_t)U * b];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // all loads before the first MFMA
+#pragma unroll
+      for (uint32_t s = 0; s < kGramChunk; ++s) {
+        const float keep = 4 * (s0 + s) + q4 < B ? 1.f : 0.f;
+        step(make_float2(ai[s].x * keep, ai[s].y * keep), aj[s]);
+      }
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[r] = make_float2(gr1[r] + gr2[r], gi1[r] + gi2[r]);
   };
   auto put_tile = [&](int mi, int mj, int l, int r, float2 v) {
     const int row = 16 * mi + 4 * (l >> 4) + r, col = 16 * mj + (l & 15);
+    if (row >= (int)U || col >= (int)U) v = make_float2(0.f, 0.f);
     sG[row * LDG + col] = v;
     if (mi != mj) sG[col * LDG + row] = make_float2(v.x, -v.y);
   };
@@ -217,73 +345,10 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
     }
   }
   __syncthreads();
-  for (int i = tid; i < UP; i += NTH) {
-    if (i >= (int)U) sG[i * LDG + i] = make_float2(1.f, 0.f);
-    sh.diag[i] = sG[i * LDG + i].x;
-  }
-  __syncthreads();
 
   // ---- 2. sweeps: sG <- -G^-1 --------------------------------------------------
-  const float tol = kPivotTol * (float)U;
-  int fail = 0;
-  float2* res = sG;  // where -G^-1 ends up
-  if constexpr (UT == 1) {
-    // One wave, in registers: lane holds column c = lane%16 of rows
-    // i = lane/16 + 4j (j = 0..3).
-    if (wave == 0) {
-      const int c = i16;
-      float2 a[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = sG[(q4 + 4 * j) * LDG + c];
-      for (int k = 0; k < (int)U; ++k) {
-        const int jk = k >> 2;
-        const float2 rowreg = jk == 0 ? a[0] : jk == 1 ? a[1] : jk == 2 ? a[2] : a[3];
-        const float2 akc = shfl2(rowreg, (k & 3) * 16 + c);   // a[k][c]
-        const float piv = __shfl(rowreg.x, (k & 3) * 16 + k, 64);
-        if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
-          fail = k + 1;
-          break;
-        }
-        const float ip = 1.0f / piv;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float2 aik = shfl2(a[j], q4 * 16 + k);       // a[i][k], i = q4 + 4j
-          a[j] = sweep_entry(a[j], aik, akc, q4 + 4 * j, c, k, ip);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) sG[(q4 + 4 * j) * LDG + c] = a[j];
-      if (lane == 0) sh.diag[0] = __int_as_float(fail);
-    }
-    __syncthreads();
-    fail = __float_as_int(sh.diag[0]);
-  } else {
-    constexpr int EPT = (UP * UP + NTH - 1) / NTH;
-    float2* cur = sh.G[0];
-    float2* nxt = sh.G[1];
-    for (uint32_t k = 0; k < U; ++k) {
-      const float piv = cur[k * LDG + k].x;
-      if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
-        fail = (int)k + 1;  // uniform: every thread read the same value
-        break;
-      }
-      const float ip = 1.0f / piv;
-#pragma unroll
-      for (int j = 0; j < EPT; ++j) {
-        const int e = tid + j * NTH;
-        if (e < UP * UP) {
-          const int i = e / UP, c = e % UP;
-          nxt[i * LDG + c] = sweep_entry(cur[i * LDG + c], cur[i * LDG + k],
-                                         cur[k * LDG + c], i, c, (int)k, ip);
-        }
-      }
-      __syncthreads();
-      float2* t = cur;
-      cur = nxt;
-      nxt = t;
-    }
-    res = cur;
-  }
+  const int fail = zf_sweep<UT, NTH>(sh, U);
+  const float2* res = sG;  // where -G^-1 ends up
   if (fail) {
     for (uint32_t e = tid; e < U * B; e += NTH) {
       const uint32_t u = e % U, b = e / U;

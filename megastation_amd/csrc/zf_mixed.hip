@@ -1,0 +1,417 @@
+// Mixed-precision zero-forcing weights (the precision form of the detection
+// path): per subcarrier group, one workgroup of 256 threads computes
+//   1. G = A A^H on the BF16 matrix cores with split operands (prec_core.h),
+//      FP32 accumulation, mirrored to an exactly Hermitian matrix;
+//   2. X0 = G^-1 by the FP32 sweep operator (zf_core.h), and the 1-norm
+//      condition number kappa = |G|_1 |X0|_1 (exact: X0 is explicit);
+//   3. one step of iterative refinement of the inverse,
+//        R = I - G X0 (products and sums in FP64),  X = X0 + X0 R (FP32);
+//   4. W = X A on the BF16 matrix cores with split operands.
+// Groups with kappa above kCondMax, or whose FP32 sweep meets a pivot at or
+// below the relative tolerance, take the FP64 route instead: Gram, sweep and
+// W = G^-1 A all in FP64 (vector ALU), rounded to FP32 at the end; a pivot
+// at or below U * 2^-52 * G_kk there marks the group singular (status k+1,
+// W = 0).
+//
+// Outputs: W in the CSI layout (FP32, optional; may be the input buffer),
+// W as split BF16 words for the mixed equalize kernel (optional;
+// Wp[u + U b] = {hi pair, lo pair}, see prec_core.h), the status and the
+// route (1 = FP64) per group.
+#include "prec_core.h"
+#include "zf_core.h"
+
+namespace mega {
+namespace detail {
+
+constexpr float kCondMax = 1e3f;  // kappa_1(G) above this takes the FP64 route
+
+namespace {
+
+constexpr int kMixThreads = 256;
+
+template <int UT>
+struct ZfMixShared {
+  static constexpr int UP = 16 * UT;
+  static constexpr int LDG = UP + 1;
+  ZfShared<UT> sw;     // G, then -G^-1 (sweep); later X (refined inverse)
+  float2 G0[UP * LDG];  // copy of G for the residual; later R; later split X
+  float red[2];
+};
+
+__device__ __forceinline__ float cabs2(float2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+
+// max over columns c < U of sum_i |M[i][c]|, by wave 0 (U <= 64); result in
+// *out after the caller's next barrier.
+template <int LDG>
+__device__ __forceinline__ void col_norm1(const float2* M, uint32_t U, float* out) {
+  if (threadIdx.x >= 64) return;
+  const uint32_t c = threadIdx.x;
+  float s = 0.f;
+  if (c < U)
+    for (uint32_t i = 0; i < U; ++i) s += cabs2(M[i * LDG + c]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s = fmaxf(s, __shfl_xor(s, off, 64));
+  if (c == 0) *out = s;
+}
+
+// FP64 sweep on M (UP x LDG, Hermitian, padded with identity) -> -M^-1, one
+// barrier per pivot; entries in registers, pivot row/column through prow/pcol
+// (2 x UP each). Returns 0 or k+1. Ends with a barrier.
+template <int UP, int LDG>
+__device__ int sweep_f64(double2* M, double2* prow, double2* pcol, double* diag,
+                         uint32_t U) {
+  constexpr int NTH = kMixThreads;
+  constexpr int EPT = UP * UP / NTH;
+  constexpr int RS = NTH / UP;
+  static_assert((UP * UP) % NTH == 0 && NTH % UP == 0, "sweep layout");
+  const int tid = threadIdx.x;
+  const int c = tid % UP, r0 = tid / UP;
+  for (int i = tid; i < UP; i += NTH) {
+    if (i >= (int)U) M[i * LDG + i] = make_double2(1.0, 0.0);
+    diag[i] = M[i * LDG + i].x;
+  }
+  __syncthreads();
+  double2 a[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) a[j] = M[(r0 + RS * j) * LDG + c];
+  auto publish = [&](int k, int buf) {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int i = r0 + RS * j;
+      if (i == k) prow[buf * UP + c] = a[j];
+      if (c == k) pcol[buf * UP + i] = a[j];
+    }
+  };
+  publish(0, 0);
+  __syncthreads();
+  const double tol = 0x1p-52 * (double)U;
+  int fail = 0;
+  for (uint32_t k = 0; k < U; ++k) {
+    const int buf = k & 1;
+    const double piv = prow[buf * UP + k].x;
+    if (!(piv > tol * diag[k]) || !isfinite(piv)) {
+      fail = (int)k + 1;
+      break;
+    }
+    const double ip = 1.0 / piv;
+    const double2 akc = prow[buf * UP + c];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int i = r0 + RS * j;
+      const double2 aik = pcol[buf * UP + i];
+      const double2 base = (i == (int)k || c == (int)k) ? make_double2(0.0, 0.0) : a[j];
+      const double2 x = (i == (int)k) ? make_double2(-1.0, 0.0) : aik;
+      const double2 y = (c == (int)k) ? make_double2(-1.0, 0.0) : akc;
+      const double pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;
+      a[j] = make_double2(base.x - pr * ip, base.y - pi * ip);
+    }
+    if (k + 1 < U) publish((int)k + 1, buf ^ 1);
+    __syncthreads();
+  }
+  if (!fail) {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) M[(r0 + RS * j) * LDG + c] = a[j];
+  }
+  __syncthreads();
+  return fail;
+}
+
+__device__ __forceinline__ void store_w(float2* w, uint2* wp, uint64_t idx, float2 v) {
+  if (w) w[idx] = v;
+  if (wp) {
+    const SplitC s = split_c(v);
+    wp[idx] = make_uint2(s.hi, s.lo);
+  }
+}
+
+template <int UT>
+__global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict__ in,
+                                                        float2* w_out, uint2* wp_out,
+                                                        int32_t* info, int32_t* route,
+                                                        uint32_t U, uint32_t B) {
+  constexpr int UP = 16 * UT;
+  constexpr int LDG = UP + 1;
+  constexpr int NTH = kMixThreads;
+  constexpr int NW = NTH / 64;
+  __shared__ ZfMixShared<UT> sh;
+  extern __shared__ double2 sM64[];  // FP64 route: UP x LDG, then prow/pcol, diag
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const bool h = i16 & 1;
+  const uint32_t g = blockIdx.x;
+  const float2* A = in + (uint64_t)g * U * B;
+  float2* w = w_out ? w_out + (uint64_t)g * U * B : nullptr;
+  uint2* wp = wp_out ? wp_out + (uint64_t)g * U * B : nullptr;
+  float2* sG = sh.sw.G;
+  const uint32_t nst = (B + 15) / 16;
+
+  // ---- 1. Gram on split BF16 products: tiles of 8 (rows) x 16 (cols) users --
+  // Antenna steps of 16 stream through a ring of RG register sets (RG-1
+  // steps in flight); loads past the last step re-read it (clamped) and are
+  // masked to zero like antennas past B.
+  {
+    constexpr int RT = 2 * UT, CT = UT, RG = 4;
+    for (int tile = wave; tile < RT * CT; tile += NW) {
+      const int rt = tile / CT, ct = tile % CT;
+      const uint32_t u = min(8u * rt + (i16 >> 1), U - 1);
+      const uint32_t v = min(16u * ct + i16, U - 1);
+      float2 zu[RG][4], zv[RG][4];
+      auto load = [&](uint32_t st, float2 (&du)[4], float2 (&dv)[4]) {
+        st = min(st, nst - 1);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const uint32_t b = min(16 * st + 4 * q4 + a, B - 1);
+          du[a] = A[u + (uint64_t)U * b];
+          dv[a] = A[v + (uint64_t)U * b];
+        }
+      };
+#pragma unroll
+      for (int r = 0; r + 1 < RG; ++r) load(r, zu[r], zv[r]);
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+      for (uint32_t s0 = 0; s0 < nst; s0 += RG) {
+#pragma unroll
+        for (int r = 0; r < RG; ++r) {
+          const uint32_t st = s0 + r;
+          load(st + RG - 1, zu[(r + RG - 1) % RG], zv[(r + RG - 1) % RG]);
+          __builtin_amdgcn_sched_barrier(0);
+          v4u ah, al, bh, bl;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const bool ok = 16 * st + 4 * q4 + a < B;
+            const SplitC su = split_c(ok ? zu[r][a] : make_float2(0.f, 0.f));
+            const SplitC sv = split_c(zv[r][a]);
+            ah[a] = a_word(su.hi, h);
+            al[a] = a_word(su.lo, h);
+            bh[a] = sv.hi ^ 0x80000000u;  // conj(A[v][b])
+            bl[a] = sv.lo ^ 0x80000000u;
+          }
+          acc = mfma_x3(ah, al, bh, bl, acc);
+        }
+      }
+      const uint32_t col = 16 * ct + i16;
+#pragma unroll
+      for (int r2 = 0; r2 < 2; ++r2) {
+        const uint32_t row = 8 * rt + 2 * q4 + r2;
+        const bool ok = row < U && col < U;
+        sG[row * LDG + col] = ok ? make_float2(acc[2 * r2], acc[2 * r2 + 1]) : make_float2(0.f, 0.f);
+      }
+    }
+  }
+  __syncthreads();
+  // exact Hermitian symmetry (lower triangle mirrored) and the copy for R
+  for (int e = tid; e < UP * UP; e += NTH) {
+    const int i = e / UP, c = e % UP;
+    float2 val = sG[i * LDG + c];
+    if (i < c) {
+      const float2 t = sG[c * LDG + i];
+      val = make_float2(t.x, -t.y);
+    }
+    if (i == c) val.y = 0.f;
+    sG[i * LDG + c] = val;
+    sh.G0[i * LDG + c] = val;
+  }
+  __syncthreads();
+  col_norm1<LDG>(sh.G0, U, &sh.red[0]);
+
+  // ---- 2. FP32 sweep and condition number ------------------------------------
+  int fail = zf_sweep<UT, NTH>(sh.sw, U);  // starts with the barrier it needs
+  bool fp64 = fail != 0;
+  if (!fp64) {
+    col_norm1<LDG>(sG, U, &sh.red[1]);
+    __syncthreads();
+    const float kappa = sh.red[0] * sh.red[1];
+    fp64 = !(kappa <= kCondMax);
+  }
+
+  if (!fp64) {
+    // ---- 3. refinement: R = I - G X0 (FP64), X = X0 + X0 R -----------------
+    constexpr int EPT = UP * UP / NTH;
+    constexpr int RS = NTH / UP;
+    const int c = tid % UP, r0 = tid / UP;
+    float2 rr[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int i = r0 + RS * j;
+      double sr = (i == c) ? 1.0 : 0.0, si = 0.0;
+      for (uint32_t k = 0; k < U; ++k) {
+        const float2 gk = sh.G0[i * LDG + k];
+        const float2 xk = sG[k * LDG + c];  // -X0
+        sr += (double)gk.x * xk.x - (double)gk.y * xk.y;
+        si += (double)gk.x * xk.y + (double)gk.y * xk.x;
+      }
+      rr[j] = (i < (int)U && c < (int)U) ? make_float2((float)sr, (float)si) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) sh.G0[(r0 + RS * j) * LDG + c] = rr[j];
+    __syncthreads();
+    float2 xn[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int i = r0 + RS * j;
+      const float2 x0 = sG[i * LDG + c];
+      float xr = -x0.x, xi = -x0.y;
+      for (uint32_t k = 0; k < U; ++k) {
+        const float2 xk = sG[i * LDG + k];
+        const float2 rk = sh.G0[k * LDG + c];
+        xr -= xk.x * rk.x - xk.y * rk.y;
+        xi -= xk.x * rk.y + xk.y * rk.x;
+      }
+      xn[j] = make_float2(xr, xi);
+    }
+    __syncthreads();
+    // split X into BF16 words for the apply (G0 reused as uint2 storage)
+    uint2* sX = reinterpret_cast<uint2*>(sh.G0);
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const SplitC s = split_c(xn[j]);
+      sX[(r0 + RS * j) * LDG + c] = make_uint2(s.hi, s.lo);
+    }
+    __syncthreads();
+
+    // ---- 4. W = X A on split BF16 products ----------------------------------
+    // A tiles of 16 antennas x all users, double-buffered across the wave's
+    // tiles (the next tile's loads in flight during this tile's products).
+    const uint32_t ntiles = (B + 15) / 16;
+    float2 zb[2][UT][4];
+    auto load_a = [&](uint32_t nt, float2 (&d)[UT][4]) {
+      const uint32_t bc = min(16 * min(nt, ntiles - 1) + i16, B - 1);
+#pragma unroll
+      for (int ks = 0; ks < UT; ++ks)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+          d[ks][a] = A[min(16u * ks + 4 * q4 + a, U - 1) + (uint64_t)U * bc];
+    };
+    load_a(wave, zb[0]);
+    for (uint32_t n0 = wave; n0 < ntiles; n0 += 2 * NW) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const uint32_t nt = n0 + r * NW;
+        if (nt >= ntiles) break;
+        load_a(nt + NW, zb[r ^ 1]);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t b = 16 * nt + i16;
+        v4u bh[UT], bl[UT];
+#pragma unroll
+        for (int ks = 0; ks < UT; ++ks)
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const bool ok = 16u * ks + 4 * q4 + a < U;
+            const SplitC sp = split_c(ok ? zb[r][ks][a] : make_float2(0.f, 0.f));
+            bh[ks][a] = sp.hi;
+            bl[ks][a] = sp.lo;
+          }
+#pragma unroll
+        for (int mt = 0; mt < 2 * UT; ++mt) {
+          const uint32_t u = 8 * mt + (i16 >> 1);
+          v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int ks = 0; ks < UT; ++ks) {
+            v4u ah, al;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+              const uint2 xw = sX[u * LDG + 16 * ks + 4 * q4 + a];
+              ah[a] = a_word(xw.x, h);
+              al[a] = a_word(xw.y, h);
+            }
+            acc = mfma_x3(ah, al, bh[ks], bl[ks], acc);
+          }
+          if (b < B) {
+            const uint32_t u0 = 8 * mt + 2 * q4;
+            if (u0 < U) store_w(w, wp, u0 + (uint64_t)U * b, make_float2(acc[0], acc[1]));
+            if (u0 + 1 < U) store_w(w, wp, u0 + 1 + (uint64_t)U * b, make_float2(acc[2], acc[3]));
+          }
+        }
+      }
+    }
+  } else {
+    // ---- FP64 route ------------------------------------------------------------
+    double2* M = sM64;
+    double2* prow = sM64 + UP * LDG;
+    double2* pcol = prow + 2 * UP;
+    double* diag = reinterpret_cast<double*>(pcol + 2 * UP);
+    for (int e = tid; e < UP * UP; e += NTH) {
+      const uint32_t i = e / UP, c = e % UP;
+      double sr = 0.0, si = 0.0;
+      if (i < U && c < U && i >= c) {
+        for (uint32_t b = 0; b < B; ++b) {
+          const float2 x = A[i + (uint64_t)U * b], y = A[c + (uint64_t)U * b];
+          sr += (double)x.x * y.x + (double)x.y * y.y;
+          si += (double)x.y * y.x - (double)x.x * y.y;
+        }
+      }
+      M[i * LDG + c] = make_double2(sr, si);
+    }
+    __syncthreads();
+    for (int e = tid; e < UP * UP; e += NTH) {
+      const int i = e / UP, c = e % UP;
+      if (i < c) {
+        const double2 t = M[c * LDG + i];
+        M[i * LDG + c] = make_double2(t.x, -t.y);
+      }
+    }
+    __syncthreads();
+    fail = sweep_f64<UP, LDG>(M, prow, pcol, diag, U);
+    // W = G^-1 A in rounds of NTH weights, each read before any is written
+    // (w may alias A).
+    for (uint32_t base = 0; base < U * B; base += NTH) {
+      const uint32_t e = base + tid;
+      double sr = 0.0, si = 0.0;
+      if (!fail && e < U * B) {
+        const uint32_t u = e % U, b = e / U;
+        for (uint32_t v = 0; v < U; ++v) {
+          const double2 x = M[u * LDG + v];  // -G^-1
+          const float2 z = A[v + (uint64_t)U * b];
+          sr -= x.x * z.x - x.y * z.y;
+          si -= x.x * z.y + x.y * z.x;
+        }
+      }
+      __syncthreads();
+      if (e < U * B) store_w(w, wp, e, make_float2((float)sr, (float)si));
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    if (info) info[g] = fail;
+    if (route) route[g] = fp64 ? 1 : 0;
+  }
+}
+
+template <int UT>
+hipError_t launch_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                     float2* w_out, uint2* wp, int32_t* info, int32_t* route,
+                     hipStream_t st) {
+  constexpr int UP = 16 * UT;
+  const size_t lds = (size_t)UP * (UP + 1) * sizeof(double2) + 4 * UP * sizeof(double2) +
+                     UP * sizeof(double);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zf_mixed<UT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((zf_mixed<UT>), dim3(batch), dim3(kMixThreads), lds, st, in, w_out, wp,
+                     info, route, U, B);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_zf_mixed(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                           float2* w_out, void* wp, int32_t* info, int32_t* route,
+                           hipStream_t st, const char** why) {
+  if (batch == 0) return hipSuccess;
+  if (batch > 0x7fffffffu) {
+    *why = "batch too large";
+    return hipErrorInvalidValue;
+  }
+  uint2* w2 = static_cast<uint2*>(wp);
+  if (U <= 16) return launch_ut<1>(U, B, batch, in, w_out, w2, info, route, st);
+  if (U <= 32) return launch_ut<2>(U, B, batch, in, w_out, w2, info, route, st);
+  if (U <= 64) return launch_ut<4>(U, B, batch, in, w_out, w2, info, route, st);
+  *why = "more than 64 users";
+  return hipErrorInvalidValue;
+}
+
+}  // namespace detail
+}  // namespace mega

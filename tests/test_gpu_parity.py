@@ -147,9 +147,13 @@ EQ_CASES = [
 ]
 
 
+# MEGA_EQ_MODE: 0 = the launcher's choice, 1 = W streamed from memory,
+# 2 = W held in registers (antennas split over waves when B is wide)
+@pytest.mark.parametrize("mode", ["0", "1", "2"])
 @pytest.mark.parametrize("U,B,N,sg,Q,layout", EQ_CASES)
-def test_equalize_matches_oracle(U, B, N, sg, Q, layout):
-    print(f"case: equalize U={U} B={B} N={N} sg={sg} Q={Q} {layout}", flush=True)
+def test_equalize_matches_oracle(U, B, N, sg, Q, layout, mode, monkeypatch):
+    print(f"case: equalize U={U} B={B} N={N} sg={sg} Q={Q} {layout} mode={mode}", flush=True)
+    monkeypatch.setenv("MEGA_EQ_MODE", mode)
     S, slots = 2, 1
     sl = make_slots(U * 7 + N, slots, U, B, N, sg, S, Q)
     lay = synth.ldpc_uplink_layout(N, Q) if layout == "ldpc" else synth.standalone_layout(N, Q)
@@ -280,10 +284,12 @@ def _detect(csi, y, U, B, N, sg, Q, lay, slots, S, want_w=True, want_x=True):
                                         (16, 128, 600, 16, 6), (12, 192, 300, 12, 2),
                                         (16, 256, 240, 16, 4), (5, 24, 100, 7, 6),
                                         (32, 128, 480, 16, 6)])
-def test_detect_slots_equals_zf_then_equalize(U, B, N, sg, Q):
-    """Fused kernel == ZF kernel followed by the equalize kernel, bit for bit
-    (same arithmetic), and x_hat / EVM against the FP64 oracle."""
+def test_detect_slots_equals_zf_then_equalize(U, B, N, sg, Q, monkeypatch):
+    """Fused kernel == ZF kernel followed by the equalize kernel with W in
+    registers, bit for bit (same arithmetic and summation order), and x_hat /
+    EVM against the FP64 oracle."""
     print(f"case: detect U={U} B={B} N={N} sg={sg} Q={Q}", flush=True)
+    monkeypatch.setenv("MEGA_EQ_MODE", "2")
     S, slots = 3, 2
     sl = make_slots(U * 31 + B, slots, U, B, N, sg, S, Q, 25.0)
     lay = synth.standalone_layout(N, Q)
