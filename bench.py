@@ -133,7 +133,8 @@ def main():
         if evs:
             evs[2].record(stream)
 
-    split = torch.empty(slots * G * B * U, dtype=torch.int64, device=dev)
+    Bp = (B + 15) // 16 * 16
+    split = torch.empty(slots * G * U * Bp, dtype=torch.int64, device=dev)
 
     def step_mixed(evs=None):
         """Mixed form: split-BF16 matrix-core ZF (split weight words to HBM),
@@ -143,7 +144,8 @@ def main():
         m.zf_split(csi, split, B, U, slots * G, stream=stream)
         if evs:
             evs[1].record(stream)
-        m.equalize_slots(split, y, llr, form=m.FORM_MIXED, stream=stream, **eq_kw)
+        m.equalize_slots(split, y, llr, form=m.FORM_MIXED, stream=stream,
+                         **dict(eq_kw, csi_slot_stride=G * U * Bp))
         if evs:
             evs[2].record(stream)
 
@@ -215,7 +217,7 @@ def main():
         else:
             m.zf_split(csi[0], split, B, U, G, stream=stream)
             m.equalize_slots(split, y[0], llr1, xhat=xh, form=m.FORM_MIXED,
-                             stream=stream, **kw1)
+                             stream=stream, **dict(kw1, csi_slot_stride=G * U * Bp))
         torch.cuda.synchronize()
         evm[name] = synth.evm_db(xh.cpu().numpy().reshape(S, U, N), x0)
     evm64 = fp64_evm(cfg, csi, y, x0)

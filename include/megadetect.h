@@ -102,11 +102,14 @@ int mega_zf(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
             mega_stream_t stream);
 
 /* Mixed-form ZF (MEGA_FORM_MIXED) emitting the weights as split BF16 words
- * for mega_equalize_slots with form MEGA_FORM_MIXED: w_split receives
- * batch * B * U 8-byte words in the CSI layout ({hi, lo} pairs of packed
- * (re, im) BF16). w_out (optional) receives W in FP32, route_dev (optional)
- * 1 for groups computed on the FP64 route. No reference counterpart: the
- * first half of mega_detect_slots with eq.form = MEGA_FORM_MIXED. */
+ * for mega_equalize_slots with form MEGA_FORM_MIXED. w_split receives, per
+ * group, a plane of hi words then a plane of lo words, each U rows of
+ * Bp = B rounded up to a multiple of 16 words; a word packs (re, im) as two
+ * BF16 (re in the low half), W = hi + lo, zeros past B: batch * U * Bp * 8
+ * bytes. w_out (optional) receives W in FP32 (CSI layout), route_dev
+ * (optional) 1 for groups computed on the FP64 route. No reference
+ * counterpart: the first half of mega_detect_slots with eq.form =
+ * MEGA_FORM_MIXED. */
 int mega_zf_split(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
                   const void *csi_in, void *w_split, void *w_out,
                   int32_t *info_dev, int32_t *route_dev, mega_stream_t stream);
@@ -136,9 +139,9 @@ int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
  *   llr  at llr + s*llr_slot_stride + t*llr_sym_stride ([U][ue_stride])
  *   xhat optional debug output, complex64 [slots][syms][U][N] (NULL = none).
  * Extends equalize.cu:20-51, which handles one symbol per launch.
- * With form = MEGA_FORM_MIXED, csi holds the split words of mega_zf_split
- * (same indexing and slot stride) and the products run on the BF16 matrix
- * cores with split operands. */
+ * With form = MEGA_FORM_MIXED, csi holds the split planes of mega_zf_split
+ * (csi_slot_stride in 8-byte units: G * U * Bp for consecutive slots) and
+ * the products run on the BF16 matrix cores with split operands. */
 typedef struct {
   uint32_t num_ue, num_bs, num_sc, sc_group; /* U, B, N, sg */
   uint32_t num_syms, num_slots;
@@ -180,7 +183,8 @@ typedef struct {
 int mega_detect_slots(const mega_detect_desc *desc, mega_stream_t stream);
 
 /* Bytes of device scratch mega_detect_slots needs for desc (0 for
- * MEGA_FORM_FP32; num_slots * G * B * U * 8 for MEGA_FORM_MIXED). */
+ * MEGA_FORM_FP32; num_slots * G * U * Bp * 8 for MEGA_FORM_MIXED, Bp = B
+ * rounded up to a multiple of 16). */
 uint64_t mega_detect_work_bytes(const mega_detect_desc *desc);
 
 /* ---- host demodulation ----------------------------------------------------

@@ -259,7 +259,8 @@ int mega_detect_slots(const mega_detect_desc* d, mega_stream_t stream) {
 uint64_t mega_detect_work_bytes(const mega_detect_desc* d) {
   if (!d || d->eq.form != MEGA_FORM_MIXED || d->eq.sc_group == 0) return 0;
   const uint64_t G = (d->eq.num_sc + d->eq.sc_group - 1) / d->eq.sc_group;
-  return (uint64_t)d->eq.num_slots * G * d->eq.num_bs * d->eq.num_ue * 8;
+  const uint64_t Bp = (d->eq.num_bs + 15) / 16 * 16;  // split planes' row length
+  return (uint64_t)d->eq.num_slots * G * Bp * d->eq.num_ue * 8;
 }
 
 int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,

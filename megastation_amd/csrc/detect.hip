@@ -171,15 +171,16 @@ hipError_t launch_detect_mixed(const EqualizeArgs& a, float2* w_out, void* work,
     *why = "the mixed form needs a work buffer";
     return hipErrorInvalidValue;
   }
-  const uint64_t per_slot = (uint64_t)a.G * a.U * a.B;  // weights per slot (dense work)
+  // split planes per slot, in 8-byte units (dense work buffer)
+  const uint64_t per_slot = (uint64_t)a.G * a.U * ((a.B + 15) / 16 * 16);
   hipError_t e = hipSuccess;
-  if (a.csi_slot == per_slot || a.slots == 1) {
+  if (a.csi_slot == (uint64_t)a.G * a.U * a.B || a.slots == 1) {
     e = launch_zf_mixed(a.U, a.B, a.slots * a.G, a.csi, w_out, work, info, route, st, why);
   } else {
     for (uint32_t s = 0; s < a.slots && e == hipSuccess; ++s)
       e = launch_zf_mixed(a.U, a.B, a.G, a.csi + s * a.csi_slot,
                           w_out ? w_out + s * a.csi_slot : nullptr,
-                          static_cast<uint2*>(work) + s * per_slot,
+                          static_cast<uint2*>(work) + s * per_slot,  // 2 words per unit
                           info ? info + (uint64_t)s * a.G : nullptr,
                           route ? route + (uint64_t)s * a.G : nullptr, st, why);
   }

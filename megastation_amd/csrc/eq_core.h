@@ -291,19 +291,19 @@ __device__ __forceinline__ void eq_tiles(const EqParams& p, const WSrc& ws,
 // One column tile t of group g for users 8*MT*mg ... 8*MT*(mg+1)-1, by one
 // wave over all B antennas, with W streamed from memory instead of held: for
 // groups whose W slice does not fit a wave's registers. The antennas go in
-// chunks of KC steps; y (the HBM stream) is kept RY-1 chunks ahead in a ring
-// of RY register sets, W (re-read by every tile of the group, so mostly L2
-// hits) one chunk ahead. No LDS and no barriers: each wave is independent.
+// chunks of KC steps through a ring of R register sets: y (the HBM stream)
+// and W (re-read by every tile of the group, so mostly L2 hits) are both
+// issued R-1 chunks ahead of their use. No LDS and no barriers: each wave is
+// independent.
 //
-// Requires B % (4*KC*RY) == 0 (eq_stream_ok). Every load is then in bounds
+// Requires B % (4*KC*R) == 0 (eq_stream_ok). Every load is then in bounds
 // without a mask: columns past the group's end re-read its last column and
 // users past U re-read user U-1 (their results are never stored), and the
 // look-ahead past the last chunk re-reads that chunk. With no branches or
 // selects around the loads the compiler waits for exactly the chunk in use.
-template <int MT, int KC, int RY, int Q>
+template <int MT, int KC, int R, int NCH, int Q>
 __device__ __forceinline__ void eq_unit_stream(const EqParams& p, uint32_t slot,
                                                uint32_t g, uint32_t t, uint32_t mg) {
-  static_assert(RY % 2 == 0, "the W ring (2) must divide the y ring");
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, q4 = lane >> 4;
   const bool odd = i16 & 1;
@@ -318,26 +318,26 @@ __device__ __forceinline__ void eq_unit_stream(const EqParams& p, uint32_t slot,
 #pragma unroll
   for (int m = 0; m < MT; ++m)
     wrow[m] = W + min(8 * (mg * MT + m) + (i16 >> 1), U - 1) + (uint64_t)q4 * U;
-  const uint32_t nch = B / (4 * KC);
+  // NCH > 0: the chunk count is known at compile time and the chunk loop is
+  // unrolled completely, so the wait before each chunk counts exactly the
+  // loads issued after it (a loop back edge makes the compiler wait for all
+  // but the newest loads, shortening the look-ahead to one chunk).
+  const uint32_t nch = NCH > 0 ? (uint32_t)NCH : B / (4 * KC);
 
-  auto load_y = [&](uint32_t c, float2 (&dst)[KC]) {
+  auto load = [&](uint32_t c, float2 (&yd)[KC], float2 (&wd)[MT][KC]) {
     c = min(c, nch - 1);
 #pragma unroll
-    for (int s = 0; s < KC; ++s) dst[s] = ycol[(uint64_t)(4 * (c * KC + s)) * N];
-  };
-  auto load_w = [&](uint32_t c, float2 (&dst)[MT][KC]) {
-    c = min(c, nch - 1);
+    for (int s = 0; s < KC; ++s) yd[s] = ycol[(uint64_t)(4 * (c * KC + s)) * N];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int s = 0; s < KC; ++s) dst[m][s] = wrow[m][(uint64_t)(4 * (c * KC + s)) * U];
+      for (int s = 0; s < KC; ++s) wd[m][s] = wrow[m][(uint64_t)(4 * (c * KC + s)) * U];
   };
 
-  float2 yr[RY][KC];
-  float2 wr[2][MT][KC];
+  float2 yr[R][KC];
+  float2 wr[R][MT][KC];
 #pragma unroll
-  for (int r = 0; r + 1 < RY; ++r) load_y(r, yr[r]);
-  load_w(0, wr[0]);
+  for (int r = 0; r + 1 < R; ++r) load(r, yr[r], wr[r]);
 
   v4f acc1[MT], acc2[MT];
 #pragma unroll
@@ -345,23 +345,38 @@ __device__ __forceinline__ void eq_unit_stream(const EqParams& p, uint32_t slot,
     acc1[m] = v4f{0.f, 0.f, 0.f, 0.f};
     acc2[m] = v4f{0.f, 0.f, 0.f, 0.f};
   }
-  for (uint32_t c0 = 0; c0 < nch; c0 += RY) {
+  auto chunk = [&](uint32_t c, int r) {
+    load(c + R - 1, yr[(r + R - 1) % R], wr[(r + R - 1) % R]);
+    // keep the look-ahead loads ahead of this chunk's MFMAs (the scheduler
+    // would otherwise sink them next to their uses)
+    __builtin_amdgcn_sched_barrier(0);
+    // W values -> the two A operands, in place and all at once: the MFMAs
+    // then read registers no VALU instruction is waiting to overwrite.
+    float2 a[MT][KC];
 #pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      const uint32_t c = c0 + r;
-      load_y(c + RY - 1, yr[(r + RY - 1) % RY]);
-      load_w(c + 1, wr[(r + 1) % 2]);
-      // keep the look-ahead loads ahead of this chunk's MFMAs (the scheduler
-      // would otherwise sink them next to their uses)
-      __builtin_amdgcn_sched_barrier(0);
+    for (int s = 0; s < KC; ++s)
 #pragma unroll
-      for (int s = 0; s < KC; ++s)
+      for (int m = 0; m < MT; ++m) {
+        const float2 v = wr[r][m][s];
+        a[m][s] = make_float2(odd ? v.y : v.x, odd ? v.x : -v.y);
+      }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-          const float2 v = wr[r % 2][m][s];
-          acc1[m] = mfma4(odd ? v.y : v.x, yr[r][s].x, acc1[m]);
-          acc2[m] = mfma4(odd ? v.x : -v.y, yr[r][s].y, acc2[m]);
-        }
+    for (int s = 0; s < KC; ++s)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        acc1[m] = mfma4(a[m][s].x, yr[r][s].x, acc1[m]);
+        acc2[m] = mfma4(a[m][s].y, yr[r][s].y, acc2[m]);
+      }
+  };
+  if constexpr (NCH > 0) {
+    static_assert(NCH % R == 0, "whole rings");
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) chunk(c, c % R);
+  } else {
+    for (uint32_t c0 = 0; c0 < nch; c0 += R) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) chunk(c0 + r, r);
     }
   }
   const EqCol col = eq_col<Q>(p, slot, g, t);
@@ -369,8 +384,8 @@ __device__ __forceinline__ void eq_unit_stream(const EqParams& p, uint32_t slot,
   for (int m = 0; m < MT; ++m) eq_store<Q>(p, col, 8 * (mg * MT + m) + 2 * q4, acc1[m] + acc2[m]);
 }
 
-// Shapes eq_unit_stream<MT, KC, RY, Q> takes.
-inline bool eq_stream_ok(uint32_t B, int KC, int RY) { return B % (4 * KC * RY) == 0; }
+// Shapes eq_unit_stream<MT, KC, R, Q> takes.
+inline bool eq_stream_ok(uint32_t B, int KC, int R) { return B % (4 * KC * R) == 0; }
 
 // Fills the launch-independent fields of EqParams from the C-ABI arguments
 // (equalize.hip).

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(64 * max_waves(MT, KS)) void equalize_mfma(const Eq
 // Streamed-W form: one wave per (slot, group, column tile, user group), four
 // independent waves per workgroup, units numbered so that a group's tiles
 // share an XCD (and its L2 copy of W_g).
-template <int MT, int KC, int RY, int Q>
+template <int MT, int KC, int R, int NCH, int Q>
 __global__ __launch_bounds__(256) void equalize_stream(const EqParams p, uint32_t units) {
   const uint32_t unit = xcd_linear(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (unit >= units) return;
@@ -61,29 +61,39 @@ __global__ __launch_bounds__(256) void equalize_stream(const EqParams p, uint32_
   const uint32_t rest = unit / p.nmg / p.col_tiles;
   const uint32_t g = rest % p.G, slot = rest / p.G;
   if (16 * t >= min(p.sg, p.N - g * p.sg) * p.S) return;  // past a partial group
-  eq_unit_stream<MT, KC, RY, Q>(p, slot, g, t, mg);
+  eq_unit_stream<MT, KC, R, NCH, Q>(p, slot, g, t, mg);
 }
 
-template <int MT, int KC, int RY>
+template <int MT, int KC, int R, int NCH>
 hipError_t launch_stream_q(const EqParams& p, uint32_t units, uint32_t Q, hipStream_t st) {
   const dim3 grid((units + 3) / 4), block(256);
   switch (Q) {
-    case 2: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 2>), grid, block, 0, st, p, units); break;
-    case 4: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 4>), grid, block, 0, st, p, units); break;
-    case 6: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 6>), grid, block, 0, st, p, units); break;
-    case 8: hipLaunchKernelGGL((equalize_stream<MT, KC, RY, 8>), grid, block, 0, st, p, units); break;
+    case 2: hipLaunchKernelGGL((equalize_stream<MT, KC, R, NCH, 2>), grid, block, 0, st, p, units); break;
+    case 4: hipLaunchKernelGGL((equalize_stream<MT, KC, R, NCH, 4>), grid, block, 0, st, p, units); break;
+    case 6: hipLaunchKernelGGL((equalize_stream<MT, KC, R, NCH, 6>), grid, block, 0, st, p, units); break;
+    case 8: hipLaunchKernelGGL((equalize_stream<MT, KC, R, NCH, 8>), grid, block, 0, st, p, units); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-// Chunk shape (KC antenna steps, ring of RY y chunks): 2x4 by default,
-// MEGA_EQ_SKC=4 selects 4x2 (tuning sweeps). B must be a multiple of 32.
+// Chunk shapes (KC antenna steps x ring of R chunks); all need B % 32 == 0.
+// 0: 2x4 (default), 1: 4x2, 2: 1x8 (MEGA_EQ_SCH selects, for tuning sweeps).
+// B = 64, 128, 256 get fully unrolled chunk loops (shape 0).
+constexpr int kStreamKC[3] = {2, 4, 1};
+constexpr int kStreamR[3] = {4, 2, 8};
+
 template <int MT>
-hipError_t launch_stream(const EqParams& p, uint32_t units, uint32_t Q, int kc,
+hipError_t launch_stream(const EqParams& p, uint32_t units, uint32_t Q, int shape,
                          hipStream_t st) {
-  if (kc == 4) return launch_stream_q<MT, 4, 2>(p, units, Q, st);
-  return launch_stream_q<MT, 2, 4>(p, units, Q, st);
+  if (shape == 1) return launch_stream_q<MT, 4, 2, 0>(p, units, Q, st);
+  if (shape == 2) return launch_stream_q<MT, 1, 8, 0>(p, units, Q, st);
+  switch (p.B) {
+    case 64: return launch_stream_q<MT, 2, 4, 8>(p, units, Q, st);
+    case 128: return launch_stream_q<MT, 2, 4, 16>(p, units, Q, st);
+    case 256: return launch_stream_q<MT, 2, 4, 32>(p, units, Q, st);
+    default: return launch_stream_q<MT, 2, 4, 0>(p, units, Q, st);
+  }
 }
 
 template <int MT, int KS, int Q>
@@ -167,8 +177,8 @@ hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t st,
   // W held in registers while one wave covers all antennas; otherwise W is
   // streamed (MEGA_EQ_MODE=1 forces streaming, 2 forbids it).
   const int mode = env_int("MEGA_EQ_MODE");
-  const int kc = env_int("MEGA_EQ_SKC") == 4 ? 4 : 2;
-  if (mode != 2 && (mode == 1 || pl.nwk > 1) && eq_stream_ok(a.B, kc, kc == 4 ? 2 : 4)) {
+  const int sh = env_int("MEGA_EQ_SCH") % 3;
+  if (mode != 2 && (mode == 1 || pl.nwk > 1) && eq_stream_ok(a.B, kStreamKC[sh], kStreamR[sh])) {
     uint32_t MT = a.U <= 8 ? 1 : a.U <= 16 ? 2 : 4;
     const int smt = env_int("MEGA_EQ_SMT");  // users per wave / 8 (sweeps)
     if (smt == 1 || smt == 2 || smt == 4) MT = (uint32_t)smt;
@@ -182,9 +192,9 @@ hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t st,
       return hipErrorInvalidValue;
     }
     switch (MT) {
-      case 1: return launch_stream<1>(p, (uint32_t)units, a.Q, kc, st);
-      case 2: return launch_stream<2>(p, (uint32_t)units, a.Q, kc, st);
-      default: return launch_stream<4>(p, (uint32_t)units, a.Q, kc, st);
+      case 1: return launch_stream<1>(p, (uint32_t)units, a.Q, sh, st);
+      case 2: return launch_stream<2>(p, (uint32_t)units, a.Q, sh, st);
+      default: return launch_stream<4>(p, (uint32_t)units, a.Q, sh, st);
     }
   }
   if (pl.nwk > (uint32_t)max_waves(pl.MT, pl.KS)) {

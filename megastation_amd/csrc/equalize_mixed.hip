@@ -1,95 +1,110 @@
 // Equalize + demod of the mixed form: x = W y on the BF16 matrix cores with
 // split operands (prec_core.h; FP32 accumulation), W read as the split BF16
-// words the mixed ZF kernel writes (Wp[u + U b] = {hi pair, lo pair}), y
-// split on the fly. One wave per (slot, group, 16-column tile, group of 8*MT
-// users), four independent waves per workgroup, as the streamed FP32 kernel
-// (eq_core.h); demod and LLR placement are the shared epilogue.
+// planes the mixed ZF kernel writes, y split on the fly. One wave per (slot,
+// group, 16-column tile, group of 16*MT users), four independent waves per
+// workgroup, as the streamed FP32 kernel (eq_core.h); demod and LLR
+// placement are the shared epilogue.
+#include <cstdlib>
+
 #include "prec_core.h"
 
 namespace mega {
 namespace detail {
 namespace {
 
-// Antenna steps of 16: y (the HBM stream) in a ring of RY register sets,
-// RY-1 steps ahead; W one step ahead. Loads past the last step or antenna
-// are clamped (always in bounds) and their y is zeroed, so the ring needs
-// no early exit.
-template <int MT, int RY, int Q>
-__device__ __forceinline__ void eq_unit_mixed(const EqParams& p, const uint2* __restrict__ Wp,
+// Rows of the A operand are 16 users with W's natural (re, im) words, so W
+// is used as loaded; the complex product takes two accumulators per tile,
+//   D_re = W * [Re y; -Im y],   D_im = W * [Im y; Re y]   (K = (antenna, part)),
+// whose B operands are the y words with the high half negated or the halves
+// swapped (shared by all user tiles). Lane l of D_re / D_im then holds users
+// 4*(l/16) .. +3 of its tile at column l%16.
+//
+// Antenna steps of 16 go through a ring of R register sets: y (the HBM
+// stream) and W (split planes, mostly L2 hits) both issued R-1 steps ahead.
+// Loads past the last step or antenna are clamped (always in bounds) and
+// their y is zeroed (the W planes hold zeros past B), so the ring needs no
+// early exit. NST > 0: the step count is known at compile time and the loop
+// is unrolled completely (exact waits; see eq_unit_stream).
+template <int MT, int R, int NST, int Q>
+__device__ __forceinline__ void eq_unit_mixed(const EqParams& p, const uint32_t* __restrict__ Ws,
                                               uint32_t slot, uint32_t g, uint32_t t,
                                               uint32_t mg) {
-  static_assert(RY % 2 == 0, "the W ring (2) must divide the y ring");
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, q4 = lane >> 4;
-  const bool h = i16 & 1;
   const uint32_t U = p.U, B = p.B, N = p.N;
+  const uint32_t Bp = split_row(B);
   const uint32_t k0 = g * p.sg;
   const uint32_t nk = min(p.sg, N - k0);
   const uint32_t n = min(16 * t + i16, nk * p.S - 1);
   const uint32_t sym = n / nk;
   const float2* ycol = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
-  const uint2* W = Wp + slot * p.csi_slot + (uint64_t)g * U * B;
-  const uint2* wrow[MT];
+  const uint32_t* Wg = Ws + 2 * (slot * p.csi_slot + (uint64_t)g * U * Bp);
+  const uint32_t* wrow[MT];
 #pragma unroll
-  for (int m = 0; m < MT; ++m) wrow[m] = W + min(8 * (mg * MT + m) + (i16 >> 1), U - 1);
-  const uint32_t nst = (B + 15) / 16;
+  for (int m = 0; m < MT; ++m)
+    wrow[m] = Wg + (uint64_t)min(16 * (mg * MT + m) + i16, U - 1) * Bp + 4 * q4;
+  const uint64_t lo_off = (uint64_t)U * Bp;
+  const uint32_t nst = NST > 0 ? (uint32_t)NST : Bp / 16;
 
-  auto load_y = [&](uint32_t st, float2 (&d)[4]) {
+  auto load = [&](uint32_t st, float2 (&yd)[4], v4u (&wh)[MT], v4u (&wl)[MT]) {
     st = min(st, nst - 1);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) d[a] = ycol[(uint64_t)min(16 * st + 4 * q4 + a, B - 1) * N];
-  };
-  auto load_w = [&](uint32_t st, uint2 (&d)[MT][4]) {
-    st = min(st, nst - 1);
+    for (int a = 0; a < 4; ++a) yd[a] = ycol[(uint64_t)min(16 * st + 4 * q4 + a, B - 1) * N];
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int a = 0; a < 4; ++a) d[m][a] = wrow[m][(uint64_t)U * min(16 * st + 4 * q4 + a, B - 1)];
+    for (int m = 0; m < MT; ++m) {
+      wh[m] = *reinterpret_cast<const v4u*>(wrow[m] + 16 * st);
+      wl[m] = *reinterpret_cast<const v4u*>(wrow[m] + lo_off + 16 * st);
+    }
   };
 
-  float2 yr[RY][4];
-  uint2 wr[2][MT][4];
+  float2 yr[R][4];
+  v4u wh[R][MT], wl[R][MT];
 #pragma unroll
-  for (int r = 0; r + 1 < RY; ++r) load_y(r, yr[r]);
-  load_w(0, wr[0]);
-  v4f acc[MT];
+  for (int r = 0; r + 1 < R; ++r) load(r, yr[r], wh[r], wl[r]);
+  v4f are[MT], aim[MT];
 #pragma unroll
-  for (int m = 0; m < MT; ++m) acc[m] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int m = 0; m < MT; ++m) are[m] = aim[m] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  for (uint32_t s0 = 0; s0 < nst; s0 += RY) {
+  auto step = [&](uint32_t st, int r) {
+    load(st + R - 1, yr[(r + R - 1) % R], wh[(r + R - 1) % R], wl[(r + R - 1) % R]);
+    __builtin_amdgcn_sched_barrier(0);
+    v4u b1h, b1l, b2h, b2l;
 #pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      const uint32_t st = s0 + r;
-      load_y(st + RY - 1, yr[(r + RY - 1) % RY]);
-      load_w(st + 1, wr[(r + 1) % 2]);
-      __builtin_amdgcn_sched_barrier(0);
-      v4u yh, yl;
+    for (int a = 0; a < 4; ++a) {
+      const bool ok = 16 * st + 4 * q4 + a < B;
+      const SplitC sp = split_c(ok ? yr[r][a] : make_float2(0.f, 0.f));
+      b1h[a] = sp.hi ^ 0x80000000u;
+      b1l[a] = sp.lo ^ 0x80000000u;
+      b2h[a] = __builtin_rotateleft32(sp.hi, 16);
+      b2l[a] = __builtin_rotateleft32(sp.lo, 16);
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const bool ok = 16 * st + 4 * q4 + a < B;
-        const SplitC sp = split_c(ok ? yr[r][a] : make_float2(0.f, 0.f));
-        yh[a] = sp.hi;
-        yl[a] = sp.lo;
-      }
+    for (int m = 0; m < MT; ++m) {
+      are[m] = mfma_x3(wh[r][m], wl[r][m], b1h, b1l, are[m]);
+      aim[m] = mfma_x3(wh[r][m], wl[r][m], b2h, b2l, aim[m]);
+    }
+  };
+  if constexpr (NST > 0) {
 #pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        v4u ah, al;
+    for (int st = 0; st < NST; ++st) step(st, st % R);
+  } else {
+    for (uint32_t s0 = 0; s0 < nst; s0 += R) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          ah[a] = a_word(wr[r % 2][m][a].x, h);
-          al[a] = a_word(wr[r % 2][m][a].y, h);
-        }
-        acc[m] = mfma_x3(ah, al, yh, yl, acc[m]);
-      }
+      for (int r = 0; r < R; ++r) step(s0 + r, r);
     }
   }
   const EqCol col = eq_col<Q>(p, slot, g, t);
 #pragma unroll
-  for (int m = 0; m < MT; ++m) eq_store<Q>(p, col, 8 * (mg * MT + m) + 2 * q4, acc[m]);
+  for (int m = 0; m < MT; ++m) {
+    const uint32_t u0 = 16 * (mg * MT + m) + 4 * q4;
+    eq_store<Q>(p, col, u0, v4f{are[m][0], aim[m][0], are[m][1], aim[m][1]});
+    eq_store<Q>(p, col, u0 + 2, v4f{are[m][2], aim[m][2], are[m][3], aim[m][3]});
+  }
 }
 
-template <int MT, int Q>
-__global__ __launch_bounds__(256) void equalize_mixed(const EqParams p, const uint2* Wp,
+template <int MT, int R, int NST, int Q>
+__global__ __launch_bounds__(256) void equalize_mixed(const EqParams p, const uint32_t* Ws,
                                                       uint32_t units) {
   const uint32_t unit = xcd_linear(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (unit >= units) return;
@@ -98,27 +113,46 @@ __global__ __launch_bounds__(256) void equalize_mixed(const EqParams p, const ui
   const uint32_t rest = unit / p.nmg / p.col_tiles;
   const uint32_t g = rest % p.G, slot = rest / p.G;
   if (16 * t >= min(p.sg, p.N - g * p.sg) * p.S) return;  // past a partial group
-  eq_unit_mixed<MT, 4, Q>(p, Wp, slot, g, t, mg);
+  eq_unit_mixed<MT, R, NST, Q>(p, Ws, slot, g, t, mg);
 }
 
-template <int MT>
-hipError_t launch_mt(const EqParams& p, const uint2* wp, uint32_t units, uint32_t Q,
-                     hipStream_t st) {
+template <int MT, int R, int NST>
+hipError_t launch_q(const EqParams& p, const uint32_t* wp, uint32_t units, uint32_t Q,
+                    hipStream_t st) {
   const dim3 grid((units + 3) / 4), block(256);
   switch (Q) {
-    case 2: hipLaunchKernelGGL((equalize_mixed<MT, 2>), grid, block, 0, st, p, wp, units); break;
-    case 4: hipLaunchKernelGGL((equalize_mixed<MT, 4>), grid, block, 0, st, p, wp, units); break;
-    case 6: hipLaunchKernelGGL((equalize_mixed<MT, 6>), grid, block, 0, st, p, wp, units); break;
-    case 8: hipLaunchKernelGGL((equalize_mixed<MT, 8>), grid, block, 0, st, p, wp, units); break;
+    case 2: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 2>), grid, block, 0, st, p, wp, units); break;
+    case 4: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 4>), grid, block, 0, st, p, wp, units); break;
+    case 6: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 6>), grid, block, 0, st, p, wp, units); break;
+    case 8: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 8>), grid, block, 0, st, p, wp, units); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+// Ring depth 4 (one 16-user tile per wave) or 3 (two tiles); B up to 256 in
+// whole steps gets a fully unrolled step loop.
+template <int MT>
+hipError_t launch_mt(const EqParams& p, const uint32_t* wp, uint32_t units, uint32_t Q,
+                     hipStream_t st) {
+  constexpr int R = MT == 2 ? 3 : 4;
+  switch (split_row(p.B)) {
+    case 64: return launch_q<MT, R, 4>(p, wp, units, Q, st);
+    case 128: return launch_q<MT, R, 8>(p, wp, units, Q, st);
+    case 256: return launch_q<MT, R, 16>(p, wp, units, Q, st);
+    default: return launch_q<MT, R, 0>(p, wp, units, Q, st);
+  }
+}
+
+int env_int(const char* name) {
+  const char* e = std::getenv(name);
+  return e ? std::atoi(e) : 0;
+}
+
 }  // namespace
 
-// a.csi is ignored: the weights come from wp (split words, CSI indexing and
-// a.csi_slot as the slot stride).
+// a.csi is ignored: the weights come from wp (split planes of mega_zf_split;
+// a.csi_slot in 8-byte units, G*U*split_row(B) for a dense buffer).
 hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStream_t st,
                                  const char** why) {
   if (a.U == 0 || a.B == 0 || a.N == 0 || a.sg == 0 || a.S == 0 || a.slots == 0)
@@ -126,8 +160,12 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStrea
   EqParams p;
   hipError_t e = fill_eq_params(a, p, why);
   if (e != hipSuccess) return e;
-  const uint32_t MT = a.U <= 8 ? 1 : a.U <= 16 ? 2 : 4;
-  p.nmg = (a.U + 8 * MT - 1) / (8 * MT);
+  // 16-user tiles per wave: 2 when there are more than 16 users (one y read
+  // for 32 users), else 1; MEGA_EQ_XMT=1|2 overrides (tuning sweeps).
+  uint32_t MT = a.U <= 16 ? 1 : 2;
+  const int smt = env_int("MEGA_EQ_XMT");
+  if (smt == 1 || smt == 2) MT = (uint32_t)smt;
+  p.nmg = (a.U + 16 * MT - 1) / (16 * MT);
   p.nwk = 1;
   p.chunks = 1;
   p.tiles_per_wg = p.col_tiles;
@@ -136,12 +174,9 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStrea
     *why = "grid too large";
     return hipErrorInvalidValue;
   }
-  const uint2* w = static_cast<const uint2*>(wp);
-  switch (MT) {
-    case 1: return launch_mt<1>(p, w, (uint32_t)units, a.Q, st);
-    case 2: return launch_mt<2>(p, w, (uint32_t)units, a.Q, st);
-    default: return launch_mt<4>(p, w, (uint32_t)units, a.Q, st);
-  }
+  const uint32_t* w = static_cast<const uint32_t*>(wp);
+  return MT == 1 ? launch_mt<1>(p, w, (uint32_t)units, a.Q, st)
+                 : launch_mt<2>(p, w, (uint32_t)units, a.Q, st);
 }
 
 }  // namespace detail

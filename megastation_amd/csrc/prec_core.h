@@ -32,6 +32,10 @@ typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 
+// Row length (words) of the split weight planes: B rounded up to whole
+// 16-antenna MFMA steps (the pad holds zeros).
+__host__ __device__ constexpr uint32_t split_row(uint32_t B) { return (B + 15) / 16 * 16; }
+
 __device__ __forceinline__ v4f mfma_bf16(v4u a, v4u b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
                                                  __builtin_bit_cast(v8bf, b), c, 0, 0, 0);

@@ -203,17 +203,22 @@ __device__ __forceinline__ int zf_sweep(ZfShared<UT>& sh, uint32_t U) {
     __syncthreads();
     for (uint32_t k = 0; k < U; ++k) {
       const int buf = k & 1;
+      // every LDS read of this pivot in one round trip
       const float piv = sh.prow[buf][k].x;
-      if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
+      const float dk = sh.diag[k];
+      const float2 akc = sh.prow[buf][c];
+      float2 aik[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) aik[j] = sh.pcol[buf][r0 + RS * j];
+      if (!(piv > tol * dk) || !isfinite(piv)) {
         fail = (int)k + 1;  // uniform: every thread read the same value
         break;
       }
       const float ip = 1.0f / piv;
-      const float2 akc = sh.prow[buf][c];
 #pragma unroll
       for (int j = 0; j < EPT; ++j) {
         const int i = r0 + RS * j;
-        a[j] = sweep_entry(a[j], sh.pcol[buf][i], akc, i, c, (int)k, ip);
+        a[j] = sweep_entry(a[j], aik[j], akc, i, c, (int)k, ip);
       }
       if (k + 1 < U) publish((int)k + 1, buf ^ 1);
       __syncthreads();
@@ -256,6 +261,33 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
   auto aval = [&](uint32_t u, uint32_t b) -> float2 {
     return (u < U && b < B) ? Asrc[u + (uint64_t)lda * b] : make_float2(0.f, 0.f);
   };
+  // From global memory (no staging): a chunk's loads are all issued up
+  // front (gram_fetch), with clamped (always valid) addresses and no
+  // branches. Rows past U carry clamped data and are cleared in put_tile;
+  // antennas past B are multiplied by 0 in gram_steps (the steps past the
+  // last antenna then add exact zeros, so the sum equals the one over the
+  // valid steps only).
+  struct GramBuf {
+    float2 ai[kGramChunk], aj[kGramChunk];
+  };
+  GramBuf gbuf[2];
+  auto gram_fetch = [&](int mi, int mj, uint32_t ch, GramBuf& L) {
+    const uint32_t ui = min(16u * mi + i16, U - 1), uj = min(16u * mj + i16, U - 1);
+#pragma unroll
+    for (uint32_t s = 0; s < kGramChunk; ++s) {
+      const uint32_t b = min(4 * (ch * kGramChunk + s) + q4, B - 1);
+      L.ai[s] = A[ui + (uint64_t)U * b];
+      L.aj[s] = A[uj + (uint64_t)U * b];
+    }
+  };
+  auto gram_steps = [&](const GramBuf& L, uint32_t ch, auto&& step) {
+    __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of the MFMAs
+#pragma unroll
+    for (uint32_t s = 0; s < kGramChunk; ++s) {
+      const float keep = 4 * (ch * kGramChunk + s) + q4 < B ? 1.f : 0.f;
+      step(make_float2(L.ai[s].x * keep, L.ai[s].y * keep), L.aj[s]);
+    }
+  };
   // Gram of tile (mi, mj) over antenna chunk ch: this lane's 4 entries.
   auto gram_chunk = [&](int mi, int mj, uint32_t ch, float2 (&out)[4]) {
     const uint32_t s0 = ch * kGramChunk;
@@ -276,25 +308,8 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
         step(ai, (mi == mj) ? ai : aval(uj, b));
       }
     } else {
-      // From global memory: all loads of the chunk issued up front, with
-      // clamped (always valid) addresses and no branches. Rows past U carry
-      // clamped data and are cleared in put_tile; antennas past B are
-      // multiplied by 0 (the steps past the last antenna then add exact
-      // zeros, so the sum equals the one over the valid steps only).
-      const uint32_t ui = min(16u * mi + i16, U - 1), uj = min(16u * mj + i16, U - 1);
-      float2 ai[kGramChunk], aj[kGramChunk];
-#pragma unroll
-      for (uint32_t s = 0; s < kGramChunk; ++s) {
-        const uint32_t b = min(4 * (s0 + s) + q4, B - 1);
-        ai[s] = A[ui + (uint64_t)U * b];
-        aj[s] = A[uj + (uint64_t)U * b];
-      }
-      __builtin_amdgcn_sched_barrier(0);  // all loads before the first MFMA
-#pragma unroll
-      for (uint32_t s = 0; s < kGramChunk; ++s) {
-        const float keep = 4 * (s0 + s) + q4 < B ? 1.f : 0.f;
-        step(make_float2(ai[s].x * keep, ai[s].y * keep), aj[s]);
-      }
+      gram_fetch(mi, mj, ch, gbuf[0]);
+      gram_steps(gbuf[0], ch, step);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[r] = make_float2(gr1[r] + gr2[r], gi1[r] + gi2[r]);
@@ -322,14 +337,49 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
       for (int r = 0; r < 4; ++r) put_tile(mi, mj, lane, r, sum[r]);
     }
   } else {
-    for (uint32_t item = wave; item < Plan::NT * nch; item += NW) {
-      const int tile = item / nch;
-      int mi, mj;
-      tile_of(tile, mi, mj);
-      float2 c[4];
-      gram_chunk(mi, mj, item % nch, c);
+    auto item_tile = [&](uint32_t item, int& mi, int& mj) { tile_of(item / nch, mi, mj); };
+    auto put_part = [&](uint32_t item, const v4f& r1, const v4f& r2, const v4f& i1,
+                        const v4f& i2) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) part[item * 256 + r * 64 + lane] = c[r];
+      for (int r = 0; r < 4; ++r)
+        part[item * 256 + r * 64 + lane] = make_float2(r1[r] + r2[r], i1[r] + i2[r]);
+    };
+    if constexpr (A_LDS) {
+      for (uint32_t item = wave; item < Plan::NT * nch; item += NW) {
+        int mi, mj;
+        item_tile(item, mi, mj);
+        float2 c[4];
+        gram_chunk(mi, mj, item % nch, c);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[item * 256 + r * 64 + lane] = c[r];
+      }
+    } else {
+      // the next item's loads are in flight during this item's MFMAs
+      const uint32_t nitems = Plan::NT * nch;
+      int mi, mj;
+      if ((uint32_t)wave < nitems) {
+        item_tile(wave, mi, mj);
+        gram_fetch(mi, mj, wave % nch, gbuf[0]);
+      }
+      for (uint32_t it0 = wave; it0 < nitems; it0 += 2 * NW) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const uint32_t item = it0 + r * NW;
+          if (item >= nitems) break;
+          if (item + NW < nitems) {
+            item_tile(item + NW, mi, mj);
+            gram_fetch(mi, mj, (item + NW) % nch, gbuf[r ^ 1]);
+          }
+          v4f gr1 = {0, 0, 0, 0}, gr2 = {0, 0, 0, 0}, gi1 = {0, 0, 0, 0}, gi2 = {0, 0, 0, 0};
+          gram_steps(gbuf[r], item % nch, [&](float2 ai, float2 aj) {
+            gr1 = mfma4(ai.x, aj.x, gr1);
+            gr2 = mfma4(ai.y, aj.y, gr2);
+            gi1 = mfma4(ai.y, aj.x, gi1);
+            gi2 = mfma4(-ai.x, aj.y, gi2);
+          });
+          put_part(item, gr1, gr2, gi1, gi2);
+        }
+      }
     }
     __syncthreads();
     for (int e = tid; e < Plan::NT * 256; e += NTH) {
@@ -361,11 +411,27 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
   // ---- 3. W = G^-1 A = -(res) A -------------------------------------------------
   constexpr int KSU = UP / 4;  // K steps over users
   const uint32_t nb_tiles = (B + 15) / 16;
-  for (uint32_t nb = wave; nb < nb_tiles; nb += NW) {
+  // A columns of the wave's antenna tiles, the next tile's in flight during
+  // this tile's MFMAs (global A: clamped addresses, users past U zeroed)
+  float2 abuf[2][KSU];
+  auto fetch_a = [&](uint32_t nb, float2 (&d)[KSU]) {
+    if constexpr (A_LDS) {
+#pragma unroll
+      for (int s = 0; s < KSU; ++s) d[s] = aval(4 * s + q4, 16 * nb + i16);
+    } else {
+      const uint32_t bc = min(16 * min(nb, nb_tiles - 1) + i16, B - 1);
+#pragma unroll
+      for (int s = 0; s < KSU; ++s) d[s] = A[min(4u * s + q4, U - 1) + (uint64_t)U * bc];
+    }
+  };
+  auto apply_tile = [&](uint32_t nb, const float2 (&cur)[KSU]) {
     const uint32_t b = 16 * nb + i16;
     float2 av[KSU];
 #pragma unroll
-    for (int s = 0; s < KSU; ++s) av[s] = aval(4 * s + q4, b);
+    for (int s = 0; s < KSU; ++s) {
+      const bool ok = 4u * s + q4 < U;
+      av[s] = ok ? cur[s] : make_float2(0.f, 0.f);
+    }
     v4f res_r[UT], res_i[UT];
 #pragma unroll
     for (int mi = 0; mi < UT; ++mi) {
@@ -394,6 +460,26 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
             if (w2) w2[u + (uint64_t)U * b] = v;
           }
         }
+    }
+  };
+  // A wave owns the antenna tiles wave, wave+NW, ...; only its own columns
+  // are read (ahead, from global memory) and overwritten, so W may replace A
+  // in place.
+  if constexpr (A_LDS) {
+    for (uint32_t nb = wave; nb < nb_tiles; nb += NW) {
+      fetch_a(nb, abuf[0]);
+      apply_tile(nb, abuf[0]);
+    }
+  } else {
+    fetch_a(wave, abuf[0]);
+    for (uint32_t n0 = wave; n0 < nb_tiles; n0 += 2 * NW) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const uint32_t nb = n0 + r * NW;
+        if (nb >= nb_tiles) break;
+        if (nb + NW < nb_tiles) fetch_a(nb + NW, abuf[r ^ 1]);
+        apply_tile(nb, abuf[r]);
+      }
     }
   }
   return 0;

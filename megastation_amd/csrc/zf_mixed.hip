@@ -14,8 +14,9 @@
 // W = 0).
 //
 // Outputs: W in the CSI layout (FP32, optional; may be the input buffer),
-// W as split BF16 words for the mixed equalize kernel (optional;
-// Wp[u + U b] = {hi pair, lo pair}, see prec_core.h), the status and the
+// W as split BF16 words for the mixed equalize kernel (optional; per group a
+// plane of hi words then a plane of lo words, each U rows of split_row(B)
+// packed (re, im) BF16 pairs, zero past B; prec_core.h), the status and the
 // route (1 = FP64) per group.
 #include "prec_core.h"
 #include "zf_core.h"
@@ -116,17 +117,29 @@ __device__ int sweep_f64(double2* M, double2* prow, double2* pcol, double* diag,
   return fail;
 }
 
-__device__ __forceinline__ void store_w(float2* w, uint2* wp, uint64_t idx, float2 v) {
-  if (w) w[idx] = v;
-  if (wp) {
+// W[u][b] to the FP32 CSI layout (w, optional) and to the split planes
+// (ws, optional; U rows of Bp words per plane).
+__device__ __forceinline__ void store_w(float2* w, uint32_t* ws, uint32_t U, uint32_t Bp,
+                                        uint32_t u, uint32_t b, float2 v) {
+  if (w) w[u + (uint64_t)U * b] = v;
+  if (ws) {
     const SplitC s = split_c(v);
-    wp[idx] = make_uint2(s.hi, s.lo);
+    ws[(uint64_t)u * Bp + b] = s.hi;
+    ws[(uint64_t)U * Bp + (uint64_t)u * Bp + b] = s.lo;
+  }
+}
+
+__device__ __forceinline__ void store_pad(uint32_t* ws, uint32_t U, uint32_t Bp, uint32_t u,
+                                          uint32_t b) {
+  if (ws) {
+    ws[(uint64_t)u * Bp + b] = 0u;
+    ws[(uint64_t)U * Bp + (uint64_t)u * Bp + b] = 0u;
   }
 }
 
 template <int UT>
 __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict__ in,
-                                                        float2* w_out, uint2* wp_out,
+                                                        float2* w_out, uint32_t* ws_out,
                                                         int32_t* info, int32_t* route,
                                                         uint32_t U, uint32_t B) {
   constexpr int UP = 16 * UT;
@@ -142,7 +155,8 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
   const uint32_t g = blockIdx.x;
   const float2* A = in + (uint64_t)g * U * B;
   float2* w = w_out ? w_out + (uint64_t)g * U * B : nullptr;
-  uint2* wp = wp_out ? wp_out + (uint64_t)g * U * B : nullptr;
+  const uint32_t Bp = split_row(B);
+  uint32_t* ws = ws_out ? ws_out + (uint64_t)g * 2 * U * Bp : nullptr;
   float2* sG = sh.sw.G;
   const uint32_t nst = (B + 15) / 16;
 
@@ -317,10 +331,13 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
             }
             acc = mfma_x3(ah, al, bh[ks], bl[ks], acc);
           }
+          const uint32_t u0 = 8 * mt + 2 * q4;
           if (b < B) {
-            const uint32_t u0 = 8 * mt + 2 * q4;
-            if (u0 < U) store_w(w, wp, u0 + (uint64_t)U * b, make_float2(acc[0], acc[1]));
-            if (u0 + 1 < U) store_w(w, wp, u0 + 1 + (uint64_t)U * b, make_float2(acc[2], acc[3]));
+            if (u0 < U) store_w(w, ws, U, Bp, u0, b, make_float2(acc[0], acc[1]));
+            if (u0 + 1 < U) store_w(w, ws, U, Bp, u0 + 1, b, make_float2(acc[2], acc[3]));
+          } else if (b < Bp) {
+            if (u0 < U) store_pad(ws, U, Bp, u0, b);
+            if (u0 + 1 < U) store_pad(ws, U, Bp, u0 + 1, b);
           }
         }
       }
@@ -368,9 +385,10 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
         }
       }
       __syncthreads();
-      if (e < U * B) store_w(w, wp, e, make_float2((float)sr, (float)si));
+      if (e < U * B) store_w(w, ws, U, Bp, e % U, e / U, make_float2((float)sr, (float)si));
       __syncthreads();
     }
+    for (uint32_t e = tid; e < U * (Bp - B); e += NTH) store_pad(ws, U, Bp, e % U, B + e / U);
   }
   if (tid == 0) {
     if (info) info[g] = fail;
@@ -380,7 +398,7 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
 
 template <int UT>
 hipError_t launch_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
-                     float2* w_out, uint2* wp, int32_t* info, int32_t* route,
+                     float2* w_out, uint32_t* wp, int32_t* info, int32_t* route,
                      hipStream_t st) {
   constexpr int UP = 16 * UT;
   const size_t lds = (size_t)UP * (UP + 1) * sizeof(double2) + 4 * UP * sizeof(double2) +
@@ -405,7 +423,7 @@ hipError_t launch_zf_mixed(uint32_t U, uint32_t B, uint32_t batch, const float2*
     *why = "batch too large";
     return hipErrorInvalidValue;
   }
-  uint2* w2 = static_cast<uint2*>(wp);
+  uint32_t* w2 = static_cast<uint32_t*>(wp);
   if (U <= 16) return launch_ut<1>(U, B, batch, in, w_out, w2, info, route, st);
   if (U <= 32) return launch_ut<2>(U, B, batch, in, w_out, w2, info, route, st);
   if (U <= 64) return launch_ut<4>(U, B, batch, in, w_out, w2, info, route, st);

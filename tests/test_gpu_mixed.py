@@ -200,11 +200,13 @@ def test_mixed_halves_equal_detect_slots():
     llr_a = llr_buffer(slots * S * U * lay.ue_stride)
     m.detect_slots(d_csi, d_y, llr_a, csi_slot_stride=G * B * U + pad,
                    form=m.FORM_MIXED, work=work, **kw)
-    split = t.empty(slots * G * B * U, dtype=t.int64, device="cuda:0")
+    Bp = (B + 15) // 16 * 16
+    assert work.numel() == slots * G * U * Bp * 8
+    split = t.empty(slots * G * U * Bp, dtype=t.int64, device="cuda:0")
     for s_ in range(slots):
-        m.zf_split(d_csi[s_], split[s_ * G * B * U:], B, U, G)
+        m.zf_split(d_csi[s_], split[s_ * G * U * Bp:], B, U, G)
     llr_b = llr_buffer(slots * S * U * lay.ue_stride)
-    m.equalize_slots(split, d_y, llr_b, csi_slot_stride=G * B * U, form=m.FORM_MIXED, **kw)
+    m.equalize_slots(split, d_y, llr_b, csi_slot_stride=G * U * Bp, form=m.FORM_MIXED, **kw)
     _sync()
     a, b = host_u16(llr_a), host_u16(llr_b)
     assert np.array_equal(a, b)
