@@ -187,6 +187,45 @@ int mega_detect_slots(const mega_detect_desc *desc, mega_stream_t stream);
  * rounded up to a multiple of 16). */
 uint64_t mega_detect_work_bytes(const mega_detect_desc *desc);
 
+/* ---- Batching form: many uneven cells, one graph launch ----------------------
+ * One cell of a batch: its own antenna x user size, subcarriers, group size,
+ * symbols, modulation order and LLR layout (as mega_equalize_slots for one
+ * slot). csi holds the channel estimates A_g = H_g^H ([G][B][U] complex64);
+ * w receives W ([G][B][U], required: the ZF output the equalize reads);
+ * info_dev (optional, int32[G]) the group status; xhat (optional) the
+ * equalized symbols ([S][U][N] complex64). No reference counterpart: the
+ * reference runs one Beamform + S Equalize calls per cell
+ * (process.cc:172-185); this runs all cells of a batch in one launch. */
+typedef struct {
+  uint32_t num_ue, num_bs, num_sc, sc_group, num_syms, mod_order;
+  uint64_t ue_stride, cb_size, cb_stride, zeros, valid_bits;
+  const void *csi;
+  void *w;
+  const void *y;
+  uint64_t y_sym_stride;  /* complex elements between symbols ([B][N] each) */
+  void *llr;
+  uint64_t llr_sym_stride; /* halves between symbols ([U][ue_stride] each) */
+  void *xhat;
+  int32_t *info_dev;
+} mega_cell_desc;
+
+typedef struct mega_batch *mega_batch_t;
+
+/* Plans a batch: cells are grouped into kernel classes (ZF by users rounded
+ * to 16-user tiles; equalize by antenna count), each equalize class's tile
+ * plan and the launch schedule are chosen by timing the candidates on these
+ * buffers (the autotuner: runs the batch several times, so outputs are
+ * overwritten), and the chosen launches are captured into one hipGraph.
+ * tune_path (optional) caches tuned plans across processes (a text file,
+ * appended). Buffers are bound: run the handle on the same buffers. */
+int mega_batch_create(const mega_cell_desc *cells, uint32_t num_cells,
+                      const char *tune_path, mega_batch_t *out);
+/* ZF + equalize + demod of every cell: one hipGraphLaunch on stream. */
+int mega_batch_run(mega_batch_t batch, mega_stream_t stream);
+/* Human-readable plan (classes, chosen tile plans, graph time). */
+const char *mega_batch_plan(mega_batch_t batch);
+int mega_batch_destroy(mega_batch_t batch);
+
 /* ---- host demodulation ----------------------------------------------------
  * Replaces the host side of demodQPSK / demod16QAM / demod64QAM /
  * demod256QAM (demodulation.h:30-67, __device__ __host__): n complex64

@@ -66,6 +66,22 @@ class DetectDesc(ctypes.Structure):
                 ("route_dev", ctypes.c_void_p)]
 
 
+class CellDesc(ctypes.Structure):
+    """Mirror of ``mega_cell_desc`` (include/megadetect.h)."""
+    _fields_ = [
+        ("num_ue", ctypes.c_uint32), ("num_bs", ctypes.c_uint32),
+        ("num_sc", ctypes.c_uint32), ("sc_group", ctypes.c_uint32),
+        ("num_syms", ctypes.c_uint32), ("mod_order", ctypes.c_uint32),
+        ("ue_stride", ctypes.c_uint64), ("cb_size", ctypes.c_uint64),
+        ("cb_stride", ctypes.c_uint64), ("zeros", ctypes.c_uint64),
+        ("valid_bits", ctypes.c_uint64),
+        ("csi", ctypes.c_void_p), ("w", ctypes.c_void_p), ("y", ctypes.c_void_p),
+        ("y_sym_stride", ctypes.c_uint64), ("llr", ctypes.c_void_p),
+        ("llr_sym_stride", ctypes.c_uint64), ("xhat", ctypes.c_void_p),
+        ("info_dev", ctypes.c_void_p),
+    ]
+
+
 def declared_symbols() -> list[str]:
     """Names of the functions include/megadetect.h declares."""
     text = open(HEADER_PATH).read()
@@ -101,6 +117,11 @@ def lib() -> ctypes.CDLL:
         "mega_detect_slots": ([ctypes.POINTER(DetectDesc), P], i32),
         "mega_detect_work_bytes": ([ctypes.POINTER(DetectDesc)], u64),
         "mega_demod_host": ([P, u64, u8, P], i32),
+        "mega_batch_create": ([ctypes.POINTER(CellDesc), u32, ctypes.c_char_p,
+                               ctypes.POINTER(P)], i32),
+        "mega_batch_run": ([P, P], i32),
+        "mega_batch_plan": ([P], ctypes.c_char_p),
+        "mega_batch_destroy": ([P], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -281,6 +302,53 @@ def detect_work_bytes(*, num_ue, num_bs, num_sc, sc_group, num_slots,
     d = DetectDesc()
     d.eq = e
     return int(lib().mega_detect_work_bytes(ctypes.byref(d)))
+
+
+class Batch:
+    """The batching form (mega_batch_*): many cells of uneven shapes planned,
+    autotuned and captured into one hipGraph; run() replays it.
+
+    cells: dicts with U, B, N, sg, S, Q, layout (synth.LlrLayout) and torch
+    device tensors csi ([G][B][U] complex64, A = H^H), w (same size), y
+    ([S][B][N]), llr (int16, S*U*ue_stride) and optionally xhat
+    ([S][U][N] complex64) and info (int32, G)."""
+
+    def __init__(self, cells, tune_path: str | None = None):
+        self._keep = cells
+        arr = (CellDesc * len(cells))()
+        for d, c in zip(arr, cells):
+            lay = c["layout"]
+            d.num_ue, d.num_bs, d.num_sc = c["U"], c["B"], c["N"]
+            d.sc_group, d.num_syms, d.mod_order = c["sg"], c["S"], c["Q"]
+            d.ue_stride, d.cb_size, d.cb_stride = lay.ue_stride, lay.cb_size, lay.cb_stride
+            d.zeros, d.valid_bits = lay.zeros, lay.valid_bits
+            d.csi, d.w, d.y = _ptr(c["csi"]).value, _ptr(c["w"]).value, _ptr(c["y"]).value
+            d.y_sym_stride = c["B"] * c["N"]
+            d.llr = _ptr(c["llr"]).value
+            d.llr_sym_stride = c["U"] * lay.ue_stride
+            d.xhat = _ptr(c.get("xhat")).value
+            d.info_dev = _ptr(c.get("info")).value
+        self._h = ctypes.c_void_p()
+        _check(lib().mega_batch_create(arr, len(cells),
+                                       tune_path.encode() if tune_path else None,
+                                       ctypes.byref(self._h)))
+
+    def run(self, stream=None):
+        _check(lib().mega_batch_run(self._h, _stream(stream)))
+
+    def plan(self) -> str:
+        return lib().mega_batch_plan(self._h).decode()
+
+    def close(self):
+        if self._h:
+            lib().mega_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def demod_host(xhat, mod_order: int):

@@ -9,6 +9,7 @@
 
 #include "../../include/megadetect.h"
 #include "mega/demod_core.h"
+#include "batch.h"
 #include "internal.h"
 
 namespace {
@@ -29,6 +30,8 @@ int hip_fail(hipError_t e, const char* what, const char* why = nullptr) {
 bool valid_q(uint64_t q) { return q == 2 || q == 4 || q == 6 || q == 8; }
 
 }  // namespace
+
+int mega::detail::set_error(int code, const std::string& msg) { return fail(code, msg); }
 
 struct mega_beamform {
   uint32_t num_bs, num_ue, batch_count;

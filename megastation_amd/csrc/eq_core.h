@@ -50,6 +50,7 @@ struct EqParams {
   uint64_t csi_slot, y_sym, y_slot, llr_sym, llr_slot;
   uint32_t cb_size32;  // cb_size (the host checks N*Q and cb_size fit 32 bits)
   uint32_t store32;    // every LLR offset even and the base 4-byte aligned
+  uint32_t Q;          // modulation order (used by the runtime-Q epilogue, Q = 0)
 };
 
 // Where the Q LLRs of subcarrier k go within a user row
@@ -61,33 +62,45 @@ struct LlrSlot {
   bool full;
 };
 
+// Q = 0 selects the runtime-Q form (p.Q), used where one kernel serves cells
+// of different modulation orders.
 template <int Q>
 __device__ __forceinline__ LlrSlot llr_slot(const EqParams& p, uint32_t k) {
+  const uint32_t q = Q > 0 ? (uint32_t)Q : p.Q;
   LlrSlot s;
-  s.bit = k * Q;
+  s.bit = k * q;
   const uint32_t cb = s.bit / p.cb_size32;
   s.off = (uint64_t)cb * p.cb_stride + p.zeros + (s.bit - cb * p.cb_size32);
-  s.full = p.store32 && s.bit + Q <= p.valid_bits && s.off + Q <= p.ue_stride;
+  s.full = p.store32 && s.bit + q <= p.valid_bits && s.off + q <= p.ue_stride;
   return s;
 }
 
 template <int Q>
 __device__ __forceinline__ void store_llrs(const EqParams& p, _Float16* row,
                                            const LlrSlot& s, float re, float im) {
-  _Float16 v[Q];
-  demod::demod<Q>(re, im, v);
-  _Float16* dst = row + s.off;
-  if (s.full) {
-#pragma unroll
-    for (int j = 0; j < Q / 2; ++j) {
-      uint32_t lo = __builtin_bit_cast(uint16_t, v[2 * j]);
-      uint32_t hi = __builtin_bit_cast(uint16_t, v[2 * j + 1]);
-      reinterpret_cast<uint32_t*>(dst)[j] = lo | (hi << 16);
+  if constexpr (Q == 0) {
+    switch (p.Q) {
+      case 2: store_llrs<2>(p, row, s, re, im); break;
+      case 4: store_llrs<4>(p, row, s, re, im); break;
+      case 6: store_llrs<6>(p, row, s, re, im); break;
+      default: store_llrs<8>(p, row, s, re, im); break;
     }
   } else {
+    _Float16 v[Q];
+    demod::demod<Q>(re, im, v);
+    _Float16* dst = row + s.off;
+    if (s.full) {
 #pragma unroll
-    for (int j = 0; j < Q; ++j)
-      if (s.bit + j < p.valid_bits && s.off + j < p.ue_stride) dst[j] = v[j];
+      for (int j = 0; j < Q / 2; ++j) {
+        uint32_t lo = __builtin_bit_cast(uint16_t, v[2 * j]);
+        uint32_t hi = __builtin_bit_cast(uint16_t, v[2 * j + 1]);
+        reinterpret_cast<uint32_t*>(dst)[j] = lo | (hi << 16);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < Q; ++j)
+        if (s.bit + j < p.valid_bits && s.off + j < p.ue_stride) dst[j] = v[j];
+    }
   }
 }
 

@@ -157,6 +157,7 @@ hipError_t fill_eq_params(const EqualizeArgs& a, EqParams& p, const char** why) 
   p.llr_sym = a.llr_sym;
   p.llr_slot = a.llr_slot;
   p.cb_size32 = (uint32_t)a.cb_size;
+  p.Q = a.Q;
   p.store32 = ((reinterpret_cast<uintptr_t>(a.llr) & 3) == 0) &&
               (a.ue_stride % 2 == 0) && (a.cb_stride % 2 == 0) &&
               (a.zeros % 2 == 0) && (a.cb_size % 2 == 0) &&

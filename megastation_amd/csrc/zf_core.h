@@ -301,7 +301,6 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
     if constexpr (A_LDS) {
       const uint32_t ui = 16 * mi + i16, uj = 16 * mj + i16;
       const uint32_t s1 = min(s0 + kGramChunk, ks);
-#pragma unroll 8
       for (uint32_t s = s0; s < s1; ++s) {
         const uint32_t b = 4 * s + q4;
         const float2 ai = aval(ui, b);
