@@ -107,7 +107,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2_64x16_200slots",
-                    choices=sorted(WORKLOADS) + ["c4_40cells"])
+                    choices=sorted(WORKLOADS) + ["c4_40cells", "c5_learned"])
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -117,6 +117,8 @@ def main():
     args = ap.parse_args()
     if args.workload == "c4_40cells" and args.gpus == 1:
         return main_batch(args)
+    if args.workload == "c5_learned" and args.gpus == 1:
+        return main_learned(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the detection path shards by subcarrier group with no exchange; "
@@ -303,6 +305,135 @@ def main():
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(cfg, lay, csi, y, x, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+def gen_y(A, cfg, seed):
+    """y [S][B][N] and x [S][U][N] (cuda) for channel estimates A = H^H
+    [G][B][U] (one channel per group), unit-energy QAM, AWGN at cfg's SNR."""
+    import torch
+    from megastation_amd import synth
+    U, B, N, sg, S, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "Q"))
+    G = (N + sg - 1) // sg
+    Np = G * sg
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    H = torch.conj(A).resolve_conj()                               # [G][B][U]
+    table = torch.from_numpy(synth.qam_table(Q)).to(dev)
+    x = table[torch.randint(0, 1 << Q, (S, U, Np), device=dev, generator=gen)]
+    X = x.reshape(S, U, G, sg).permute(2, 1, 0, 3).reshape(G, U, S * sg)
+    Hr, Hi = H.real.contiguous(), H.imag.contiguous()
+    Xr, Xi = X.real.contiguous(), X.imag.contiguous()
+    Y = torch.complex(torch.bmm(Hr, Xr) - torch.bmm(Hi, Xi), torch.bmm(Hr, Xi) + torch.bmm(Hi, Xr))
+    sigma = math.sqrt(10.0 ** (-cfg["snr"] / 10.0))
+    Y = Y + sigma * torch.randn(Y.shape, dtype=torch.complex64, device=dev, generator=gen)
+    y = Y.reshape(G, B, S, sg).permute(2, 1, 0, 3).reshape(S, B, Np)[..., :N].contiguous()
+    return y, x[..., :N].contiguous()
+
+
+def main_learned(args):
+    """The learned form over a slot sequence of the time-varying channel:
+    128 x 32 antennas x users, 3276 subcarriers in groups of 12, 64-QAM at
+    25 dB, 24-tap TDL (300 ns RMS) with 100 Hz Jakes Doppler, 0.5 ms slots.
+    The network is trained first (200k group samples from two generator
+    sequences, the library's kernels); a step is one slot of a held-out
+    sequence: learned ZF (Gram, network, 2 Newton-Schulz steps, judge,
+    plain-ZF fallback) + equalize/demod of 13 symbols. The plain FP32 form
+    (ZF + equalize) on the same slots is timed beside it."""
+    import torch
+    import megastation_amd as m
+    from megastation_amd import learned, synth
+    cfg = dict(U=32, B=128, N=3276, sg=12, S=13, Q=6, snr=25.0)
+    U, B, N, sg, S, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "Q"))
+    G = (N + sg - 1) // sg
+    lay = synth.standalone_layout(N, Q)
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    chans = [learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=s)) for s in (11, 12)]
+    net = learned.WarmStartNet(U, seed=0)
+    hist = learned.train(net, chans, samples=200_000)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    print(f"[bench] learned: trained in {train_s:.1f} s, loss {hist[0][1]:.4g} -> {hist[-1][1]:.4g}",
+          file=sys.stderr, flush=True)
+    test = learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=99))
+    n_slots = args.warmup + args.steps
+    slots = []
+    for k in range(n_slots):
+        A = test.slot(500 + k)
+        y, x = gen_y(A, cfg, seed=5000 + k)
+        slots.append((A, y, x))
+    llr = torch.empty(S * U * lay.ue_stride, dtype=torch.int16, device=dev)
+    w = torch.empty(G * B * U, dtype=torch.complex64, device=dev)
+    eq_kw = dict(num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, num_slots=1,
+                 mod_order=Q, ue_stride=lay.ue_stride, cb_size=lay.cb_size,
+                 cb_stride=lay.cb_stride, zeros=lay.zeros, valid_bits=lay.valid_bits,
+                 csi_slot_stride=G * B * U, y_sym_stride=B * N, y_slot_stride=S * B * N,
+                 llr_sym_stride=U * lay.ue_stride, llr_slot_stride=S * U * lay.ue_stride)
+    stream = torch.cuda.current_stream()
+
+    def run(form):
+        z = learned.LearnedZF(U, B, G, net) if form == "learned" else None
+        fb = []
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for k, (A, y, _) in enumerate(slots):
+            if k == args.warmup:
+                torch.cuda.synchronize()
+                e0.record(stream)
+            if z is not None:
+                W, _ = z.step(A, stream=stream)
+            else:
+                W = w
+                m.zf(A, W, B, U, G, stream=stream)
+            m.equalize_slots(W, y, llr, stream=stream, **eq_kw)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        rate = None
+        if z is not None:  # fallback share over the timed slots, in an untimed replay
+            z2 = learned.LearnedZF(U, B, G, net)
+            for k, (A, _, _) in enumerate(slots):
+                z2.step(A, stream=stream)
+                if k >= args.warmup:
+                    fb.append(z2.last["flag"].cpu())
+            rate = float(torch.cat(fb).float().mean())
+        return ms, rate, z
+
+    ms_plain, _, _ = run("plain")
+    ms_learned, fb_rate, z = run("learned")
+    # EVM of the last slot's learned W against the FP64 oracle's
+    A, y, x = slots[-1]
+    xh = torch.zeros(S * U * N, dtype=torch.complex64, device=dev)
+    W, _ = z.step(A)  # one more step of the same slot sequence state
+    m.equalize_slots(W, y, llr, xhat=xh, **eq_kw)
+    torch.cuda.synchronize()
+    x0 = x.cpu().numpy()
+    evm_gpu = synth.evm_db(xh.cpu().numpy().reshape(S, U, N), x0)
+    evm64 = fp64_evm(cfg, A[None], y[None], x0)
+    symbols = U * N * S
+    io = 8 * B * N * S + 3 * 8 * U * B * G + 2 * U * Q * N * S
+    result = {
+        "metric": METRIC, "value": symbols / (ms_learned * 1e-3), "unit": "symbols/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_learned,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (network bf16)", "data": "synthetic",
+        "config": {"workload": "c5_learned", "antennas": B, "users": U, "subcarriers": N,
+                   "sc_group": sg, "groups": G, "data_symbols": S, "qam": 1 << Q,
+                   "snr_db": cfg["snr"], "channel": "TDL-24 300ns, Jakes 100 Hz, 0.5 ms slots",
+                   "form": "learned", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": "whole slot (learned ZF + equalize)",
+                     "achieved": io / (ms_learned * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": io / (ms_learned * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": io, "avg_ms": ms_learned},
+        "forms": {"learned": {"ms_per_step": ms_learned, "fallback_rate": fb_rate},
+                  "plain": {"ms_per_step": ms_plain,
+                            "symbols_per_s": symbols / (ms_plain * 1e-3)}},
+        "training": {"samples": hist[-1][0], "seconds": train_s, "loss_first": hist[0][1],
+                     "loss_last": hist[-1][1]},
+        "evm": {"gpu_db": evm_gpu, "fp64_db": evm64, "delta_db": evm_gpu - evm64,
+                "within_0p1_db": abs(evm_gpu - evm64) < 0.1},
+    }
     print(json.dumps(result), flush=True)
 
 

@@ -226,6 +226,79 @@ int mega_batch_run(mega_batch_t batch, mega_stream_t stream);
 const char *mega_batch_plan(mega_batch_t batch);
 int mega_batch_destroy(mega_batch_t batch);
 
+/* ---- Learned form: network warm start for the ZF inverse --------------------
+ * Building blocks of the learned ZF of a slot sequence (no reference
+ * counterpart; the reference recomputes every slot by Cholesky). Matrices
+ * are row-major [batch][U][U] complex64; U <= 64. */
+
+/* Gram matrix G_g = A_g A_g^H and its inverse X_g by the plain FP32 sweep
+ * (the path mega_zf takes); G_out optional, X_out NULL = Gram only; a
+ * singular group gets X = 0 and status k+1 in info_dev (optional). */
+int mega_gram_inv(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const void *csi,
+                  void *G_out, void *X_out, int32_t *info_dev, mega_stream_t stream);
+
+/* Network input rows (BF16, 2*U*(U+1) per group): the upper triangles of
+ * (G - G_prev)/s and X_prev * s as (re, im) pairs, s = tr(G_prev)/U (also
+ * written to scale, float[batch]). */
+int mega_learned_features(uint32_t num_ue, uint32_t batch, const void *G, const void *G_prev,
+                          const void *X_prev, void *feat_bf16, float *scale,
+                          mega_stream_t stream);
+
+/* Training target of the network: upper((X_true - X_base) * s) as (re, im)
+ * pairs, FP32, U*(U+1) per group (X_base: the refine result with no
+ * correction and no iterations). */
+int mega_learned_target(uint32_t num_ue, uint32_t batch, const void *X_true, const void *X_base,
+                        const float *scale, float *target, mega_stream_t stream);
+
+/* X0 = X_prev - X_prev (G - G_prev) X_prev + C/s (C: the network's upper
+ * triangle, corr_ld floats per group; NULL = no correction), then ns_iters
+ * Newton-Schulz steps X <- X (2I - G X); X_out the result, res_out[g] =
+ * |X G - I|_F, flag_out[g] = 1 where it exceeds bound (or is not finite). */
+int mega_learned_refine(uint32_t num_ue, uint32_t batch, const void *G, const void *G_prev,
+                        const void *X_prev, const float *corr, uint32_t corr_ld,
+                        const float *scale, void *X_out, float *res_out, int32_t *flag_out,
+                        float bound, uint32_t ns_iters, mega_stream_t stream);
+
+/* W = X A per group (CSI layout), where flagged groups first replace X by
+ * the plain sweep inverse of their Gram matrix (the fallback); X is updated
+ * in place. info_dev (optional) gets the fallback's status. */
+int mega_learned_finish(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const void *csi,
+                        void *X, const int32_t *flag, void *w_out, int32_t *info_dev,
+                        mega_stream_t stream);
+
+/* Network kernels (BF16 operands, FP32 accumulation): C (+)= A B + bias,
+ * activation (0 identity, 1 GELU tanh form), A(i,k) = A[i*sai + k*sak],
+ * B(k,j) = B[k*sbk + j*sbj]; C row-major (ldc) in FP32 (Cf, pre-activation,
+ * Cf = z + beta*Cf) and/or BF16 (Cb, after the activation). */
+typedef struct {
+  const void *A;
+  int64_t sai, sak;
+  const void *B;
+  int64_t sbk, sbj;
+  uint32_t M, N, K;
+  const float *bias;
+  int32_t act;
+  float *Cf;
+  void *Cb;
+  uint32_t ldc;
+  float beta;
+} mega_mlp_gemm_desc;
+
+int mega_mlp_gemm(const mega_mlp_gemm_desc *d, mega_stream_t stream);
+/* dZ = dY * act'(Z) (BF16 dZ, optional FP32 copy dZf). */
+int mega_mlp_act_backward(const float *dY, const float *Z, void *dZ_bf16, float *dZf,
+                          uint64_t n, int32_t act, mega_stream_t stream);
+/* db[j] = sum_i dZ[i][j] over M rows of N. */
+int mega_mlp_col_sum(const float *dZ, float *db, uint32_t M, uint32_t N, mega_stream_t stream);
+/* dY = (Y - T) / M (gradient of the mean over M rows of 0.5 |y - t|^2). */
+int mega_mlp_mse_grad(const float *Y, const float *T, float *dY, uint64_t n, uint32_t M,
+                      mega_stream_t stream);
+/* One Adam step (bias-corrected, step >= 1) on FP32 weights w with moments
+ * m, v; wb (optional) receives the BF16 copy. */
+int mega_mlp_adam(float *w, void *wb, const float *g, float *m, float *v, uint64_t n, float lr,
+                  float beta1, float beta2, float eps, uint32_t step, mega_stream_t stream);
+int mega_mlp_to_bf16(const float *x, void *y, uint64_t n, mega_stream_t stream);
+
 /* ---- host demodulation ----------------------------------------------------
  * Replaces the host side of demodQPSK / demod16QAM / demod64QAM /
  * demod256QAM (demodulation.h:30-67, __device__ __host__): n complex64

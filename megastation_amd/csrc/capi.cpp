@@ -11,6 +11,7 @@
 #include "mega/demod_core.h"
 #include "batch.h"
 #include "internal.h"
+#include "mlp.h"
 
 namespace {
 
@@ -293,6 +294,129 @@ int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
   d.y = bsvec;
   d.llr = uevec;
   return mega_equalize_slots(&d, stream);
+}
+
+static int hip_status(hipError_t e, const char* who, const char* why = nullptr) {
+  return e == hipSuccess ? MEGA_OK : hip_fail(e, who, why);
+}
+
+int mega_gram_inv(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const void* csi,
+                  void* G_out, void* X_out, int32_t* info_dev, mega_stream_t stream) {
+  if (batch == 0) return MEGA_OK;
+  if (!csi || (!X_out && !G_out)) return fail(MEGA_ERR_INVALID_ARG, "mega_gram_inv: NULL buffer");
+  if (num_ue == 0 || num_bs == 0) return fail(MEGA_ERR_INVALID_ARG, "mega_gram_inv: zero size");
+  if (num_ue > mega::detail::kMaxUsers) return fail(MEGA_ERR_UNSUPPORTED, "mega_gram_inv: num_ue > 64");
+  return hip_status(mega::detail::launch_gram_inv(num_ue, num_bs, batch, static_cast<const float2*>(csi),
+                                                  static_cast<float2*>(G_out), static_cast<float2*>(X_out),
+                                                  info_dev, static_cast<hipStream_t>(stream)),
+                    "mega_gram_inv");
+}
+
+int mega_learned_features(uint32_t num_ue, uint32_t batch, const void* G, const void* G_prev,
+                          const void* X_prev, void* feat_bf16, float* scale, mega_stream_t stream) {
+  if (batch == 0) return MEGA_OK;
+  if (!G || !G_prev || !X_prev || !feat_bf16 || !scale)
+    return fail(MEGA_ERR_INVALID_ARG, "mega_learned_features: NULL buffer");
+  if (num_ue == 0 || num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_learned_features: num_ue outside 1..64");
+  return hip_status(mega::detail::launch_learned_features(
+                        num_ue, batch, static_cast<const float2*>(G), static_cast<const float2*>(G_prev),
+                        static_cast<const float2*>(X_prev), static_cast<__bf16*>(feat_bf16), scale,
+                        static_cast<hipStream_t>(stream)),
+                    "mega_learned_features");
+}
+
+int mega_learned_target(uint32_t num_ue, uint32_t batch, const void* X_true, const void* X_base,
+                        const float* scale, float* target, mega_stream_t stream) {
+  if (batch == 0) return MEGA_OK;
+  if (!X_true || !X_base || !scale || !target)
+    return fail(MEGA_ERR_INVALID_ARG, "mega_learned_target: NULL buffer");
+  if (num_ue == 0 || num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_learned_target: num_ue outside 1..64");
+  return hip_status(mega::detail::launch_learned_target(
+                        num_ue, batch, static_cast<const float2*>(X_true),
+                        static_cast<const float2*>(X_base), scale, target,
+                        static_cast<hipStream_t>(stream)),
+                    "mega_learned_target");
+}
+
+int mega_learned_refine(uint32_t num_ue, uint32_t batch, const void* G, const void* G_prev,
+                        const void* X_prev, const float* corr, uint32_t corr_ld, const float* scale,
+                        void* X_out, float* res_out, int32_t* flag_out, float bound,
+                        uint32_t ns_iters, mega_stream_t stream) {
+  if (batch == 0) return MEGA_OK;
+  if (!G || !G_prev || !X_prev || !X_out || !res_out || !flag_out || (corr && !scale))
+    return fail(MEGA_ERR_INVALID_ARG, "mega_learned_refine: NULL buffer");
+  if (num_ue == 0 || num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_learned_refine: num_ue outside 1..64");
+  if (corr && corr_ld < num_ue * (num_ue + 1))
+    return fail(MEGA_ERR_INVALID_ARG, "mega_learned_refine: corr_ld < U*(U+1)");
+  return hip_status(mega::detail::launch_learned_refine(
+                        num_ue, batch, static_cast<const float2*>(G), static_cast<const float2*>(G_prev),
+                        static_cast<const float2*>(X_prev), corr, corr_ld, scale,
+                        static_cast<float2*>(X_out), res_out, flag_out, bound, ns_iters,
+                        static_cast<hipStream_t>(stream)),
+                    "mega_learned_refine");
+}
+
+int mega_learned_finish(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const void* csi, void* X,
+                        const int32_t* flag, void* w_out, int32_t* info_dev, mega_stream_t stream) {
+  if (batch == 0) return MEGA_OK;
+  if (!csi || !X || !flag || !w_out) return fail(MEGA_ERR_INVALID_ARG, "mega_learned_finish: NULL buffer");
+  if (num_ue == 0 || num_bs == 0) return fail(MEGA_ERR_INVALID_ARG, "mega_learned_finish: zero size");
+  if (num_ue > mega::detail::kMaxUsers) return fail(MEGA_ERR_UNSUPPORTED, "mega_learned_finish: num_ue > 64");
+  return hip_status(mega::detail::launch_learned_finish(
+                        num_ue, num_bs, batch, static_cast<const float2*>(csi), static_cast<float2*>(X),
+                        flag, static_cast<float2*>(w_out), info_dev, static_cast<hipStream_t>(stream)),
+                    "mega_learned_finish");
+}
+
+int mega_mlp_gemm(const mega_mlp_gemm_desc* d, mega_stream_t stream) {
+  if (!d || !d->A || !d->B || (!d->Cf && !d->Cb))
+    return fail(MEGA_ERR_INVALID_ARG, "mega_mlp_gemm: NULL buffer");
+  if (d->act != 0 && d->act != 1) return fail(MEGA_ERR_UNSUPPORTED, "mega_mlp_gemm: activation");
+  if (d->M > 65535u * 64u) return fail(MEGA_ERR_UNSUPPORTED, "mega_mlp_gemm: M too large");
+  mega::detail::MlpGemm g{static_cast<const __bf16*>(d->A), d->sai, d->sak,
+                          static_cast<const __bf16*>(d->B), d->sbk, d->sbj, d->M, d->N, d->K,
+                          d->bias, d->act, d->Cf, static_cast<__bf16*>(d->Cb), d->ldc, d->beta};
+  return hip_status(mega::detail::mlp_gemm(g, static_cast<hipStream_t>(stream)), "mega_mlp_gemm");
+}
+
+int mega_mlp_act_backward(const float* dY, const float* Z, void* dZ_bf16, float* dZf, uint64_t n,
+                          int32_t act, mega_stream_t stream) {
+  if (n && (!dY || !Z || !dZ_bf16)) return fail(MEGA_ERR_INVALID_ARG, "mega_mlp_act_backward: NULL buffer");
+  return hip_status(mega::detail::mlp_act_backward(dY, Z, static_cast<__bf16*>(dZ_bf16), dZf, n, act,
+                                                   static_cast<hipStream_t>(stream)),
+                    "mega_mlp_act_backward");
+}
+
+int mega_mlp_col_sum(const float* dZ, float* db, uint32_t M, uint32_t N, mega_stream_t stream) {
+  if (N && (!dZ || !db)) return fail(MEGA_ERR_INVALID_ARG, "mega_mlp_col_sum: NULL buffer");
+  return hip_status(mega::detail::mlp_col_sum(dZ, db, M, N, static_cast<hipStream_t>(stream)),
+                    "mega_mlp_col_sum");
+}
+
+int mega_mlp_mse_grad(const float* Y, const float* T, float* dY, uint64_t n, uint32_t M,
+                      mega_stream_t stream) {
+  if (n && (!Y || !T || !dY || M == 0)) return fail(MEGA_ERR_INVALID_ARG, "mega_mlp_mse_grad: bad argument");
+  return hip_status(mega::detail::mlp_mse_grad(Y, T, dY, n, M, static_cast<hipStream_t>(stream)),
+                    "mega_mlp_mse_grad");
+}
+
+int mega_mlp_adam(float* w, void* wb, const float* g, float* m, float* v, uint64_t n, float lr,
+                  float beta1, float beta2, float eps, uint32_t step, mega_stream_t stream) {
+  if (n && (!w || !g || !m || !v || step == 0))
+    return fail(MEGA_ERR_INVALID_ARG, "mega_mlp_adam: bad argument");
+  return hip_status(mega::detail::mlp_adam(w, static_cast<__bf16*>(wb), g, m, v, n, lr, beta1, beta2,
+                                           eps, step, static_cast<hipStream_t>(stream)),
+                    "mega_mlp_adam");
+}
+
+int mega_mlp_to_bf16(const float* x, void* y, uint64_t n, mega_stream_t stream) {
+  if (n && (!x || !y)) return fail(MEGA_ERR_INVALID_ARG, "mega_mlp_to_bf16: NULL buffer");
+  return hip_status(mega::detail::mlp_to_bf16(x, static_cast<__bf16*>(y), n,
+                                              static_cast<hipStream_t>(stream)),
+                    "mega_mlp_to_bf16");
 }
 
 int mega_demod_host(const float* xhat_c64, uint64_t n, uint8_t mod_order,

@@ -51,6 +51,24 @@ hipError_t launch_detect_mixed(const EqualizeArgs& a, float2* w_out, void* work,
                                int32_t* info, int32_t* route, hipStream_t stream,
                                const char** why);
 
+// Learned form (learned.hip): Gram and plain inverse per group; network
+// features; warm start + Newton-Schulz + residual judge; final W with the
+// plain-sweep fallback. Matrices row-major [batch][U][U] complex.
+hipError_t launch_gram_inv(uint32_t U, uint32_t B, uint32_t batch, const float2* in, float2* G,
+                           float2* X, int32_t* info, hipStream_t stream);
+hipError_t launch_learned_features(uint32_t U, uint32_t batch, const float2* G, const float2* Gp,
+                                   const float2* Xp, __bf16* feat, float* scale,
+                                   hipStream_t stream);
+hipError_t launch_learned_target(uint32_t U, uint32_t batch, const float2* Xt, const float2* Xb,
+                                 const float* scale, float* tgt, hipStream_t stream);
+hipError_t launch_learned_refine(uint32_t U, uint32_t batch, const float2* G, const float2* Gp,
+                                 const float2* Xp, const float* corr, uint32_t corr_ld,
+                                 const float* scale, float2* X_out, float* res, int32_t* flag,
+                                 float bound, uint32_t iters, hipStream_t stream);
+hipError_t launch_learned_finish(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                                 float2* X, const int32_t* flag, float2* w, int32_t* info,
+                                 hipStream_t stream);
+
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,
                                hipStream_t stream);

@@ -232,16 +232,13 @@ __device__ __forceinline__ int zf_sweep(ZfShared<UT>& sh, uint32_t U) {
   return fail;
 }
 
-// Returns the group status (0 = ok, k+1 = pivot k failed), identical in every
-// thread. A is read from global memory (ld = U) or, when A_LDS, first staged
-// into sA (rows of U+1). W is written as W[u + ld_w * b] (ld_w = U for the
-// CSI layout, or a padded LDS row); w2 (optional) receives a second copy with
-// ld = U. part: LDS scratch for split Gram partials (zf_dyn_lds).
+// Gram matrix of one group into sh.G (users padded to UP with zero rows and
+// columns; the sweep puts the identity block there), ending with a barrier.
+// A is read from global memory (ld = U) or, when A_LDS, first staged into sA
+// (rows of U+1). part: LDS scratch for split Gram partials (zf_dyn_lds).
 template <int UT, int NTH, bool A_LDS>
-__device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
-                                        ZfShared<UT>& sh, float2* sA, float2* part,
-                                        float2* w, uint32_t ld_w, float2* w2) {
-  constexpr int UP = ZfShared<UT>::UP;
+__device__ __forceinline__ void zf_gram(const float2* A, uint32_t U, uint32_t B,
+                                        ZfShared<UT>& sh, float2* sA, float2* part) {
   constexpr int LDG = ZfShared<UT>::LDG;
   constexpr int NW = NTH / 64;
   using Plan = ZfPlan<UT, NW>;
@@ -253,7 +250,6 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
   const uint32_t lda = A_LDS ? U + 1 : U;
   const float2* Asrc = A_LDS ? sA : A;
   float2* sG = sh.G;
-
   if (A_LDS) {
     stage_a<NTH>(A, U, B, sA);
     __syncthreads();
@@ -395,18 +391,26 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
   }
   __syncthreads();
 
-  // ---- 2. sweeps: sG <- -G^-1 --------------------------------------------------
-  const int fail = zf_sweep<UT, NTH>(sh, U);
-  const float2* res = sG;  // where -G^-1 ends up
-  if (fail) {
-    for (uint32_t e = tid; e < U * B; e += NTH) {
-      const uint32_t u = e % U, b = e / U;
-      w[u + (uint64_t)ld_w * b] = make_float2(0.f, 0.f);
-      if (w2) w2[e] = make_float2(0.f, 0.f);
-    }
-    return fail;
-  }
+}
 
+// W = G^-1 A from -G^-1 in sh.G (the sweep result): W[u + ld_w * b], w2
+// (optional) a second copy with ld = U. A as for zf_gram (sA when A_LDS).
+template <int UT, int NTH, bool A_LDS>
+__device__ __forceinline__ void zf_apply(const float2* A, uint32_t U, uint32_t B,
+                                         ZfShared<UT>& sh, const float2* sA, float2* w,
+                                         uint32_t ld_w, float2* w2) {
+  constexpr int UP = ZfShared<UT>::UP;
+  constexpr int LDG = ZfShared<UT>::LDG;
+  constexpr int NW = NTH / 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const uint32_t lda = A_LDS ? U + 1 : U;
+  const float2* Asrc = A_LDS ? sA : A;
+  const float2* res = sh.G;
+  auto aval = [&](uint32_t u, uint32_t b) -> float2 {
+    return (u < U && b < B) ? Asrc[u + (uint64_t)lda * b] : make_float2(0.f, 0.f);
+  };
   // ---- 3. W = G^-1 A = -(res) A -------------------------------------------------
   constexpr int KSU = UP / 4;  // K steps over users
   const uint32_t nb_tiles = (B + 15) / 16;
@@ -481,6 +485,29 @@ __device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
       }
     }
   }
+}
+
+
+// Returns the group status (0 = ok, k+1 = pivot k failed), identical in every
+// thread: zf_gram, zf_sweep, zf_apply. W is written as W[u + ld_w * b] (ld_w =
+// U for the CSI layout, or a padded LDS row); w2 (optional) receives a
+// second copy with ld = U; on failure W = 0.
+template <int UT, int NTH, bool A_LDS>
+__device__ __forceinline__ int zf_group(const float2* A, uint32_t U, uint32_t B,
+                                        ZfShared<UT>& sh, float2* sA, float2* part,
+                                        float2* w, uint32_t ld_w, float2* w2) {
+  const int tid = threadIdx.x;
+  zf_gram<UT, NTH, A_LDS>(A, U, B, sh, sA, part);
+  const int fail = zf_sweep<UT, NTH>(sh, U);
+  if (fail) {
+    for (uint32_t e = tid; e < U * B; e += NTH) {
+      const uint32_t u = e % U, b = e / U;
+      w[u + (uint64_t)ld_w * b] = make_float2(0.f, 0.f);
+      if (w2) w2[e] = make_float2(0.f, 0.f);
+    }
+    return fail;
+  }
+  zf_apply<UT, NTH, A_LDS>(A, U, B, sh, sA, w, ld_w, w2);
   return 0;
 }
 

@@ -125,6 +125,75 @@ def channel_kronecker(rng, G: int, B: int, U: int, rho_rx: float,
     return (Rr[None] @ Hw @ Rt[None]).astype(np.complex64)
 
 
+class TdlJakes:
+    """Time-varying tapped-delay-line channel for sequences of slots.
+
+    n_taps taps at spacing tap_ns with an exponential power-delay profile
+    whose discrete RMS delay spread is rms_ns (the decay constant is solved
+    for that), powers summing to 1; every (antenna, user, tap) path fades
+    with Jakes' Doppler spectrum at fd_hz, built as a sum of n_sin sinusoids
+    with equally spaced arrival angles (random offset) and random phases
+    (unit mean power). Slot t samples the paths at t * slot_s; the group
+    response is the tap sum at the group's centre subcarrier,
+    H_g[b][u] = sum_l h_l[b][u](t) exp(-2 pi i f_g tau_l),
+    f_g = (g*sg + sg/2) * scs. slot(t) returns A = H^H as [G][B][U]."""
+
+    def __init__(self, B: int, U: int, N: int, sg: int, seed: int, n_taps: int = 24,
+                 rms_ns: float = 300.0, tap_ns: float = 65.0, fd_hz: float = 100.0,
+                 slot_s: float = 0.5e-3, scs_hz: float = 30e3, n_sin: int = 16):
+        self.B, self.U, self.N, self.sg = B, U, N, sg
+        self.G = (N + sg - 1) // sg
+        self.slot_s, self.fd = slot_s, fd_hz
+        tau = np.arange(n_taps) * tap_ns * 1e-9
+        self.tau = tau
+        self.power = self._pdp(tau, rms_ns * 1e-9)
+        rng = np.random.Generator(np.random.Philox(seed))
+        P = B * U * n_taps
+        m = np.arange(n_sin)
+        theta = rng.uniform(0, 2 * np.pi, size=(P, 1))
+        alpha = (2 * np.pi * m[None, :] + theta) / n_sin           # arrival angles
+        self.omega = 2 * np.pi * fd_hz * np.cos(alpha)               # [P][n_sin]
+        self.phi = rng.uniform(0, 2 * np.pi, size=(P, n_sin))
+        fg = (np.arange(self.G) * sg + sg / 2.0) * scs_hz
+        # [n_taps][G] tap phase at each group centre
+        self.steer = np.exp(-2j * np.pi * tau[:, None] * fg[None, :])
+        self.amp = np.sqrt(np.repeat(self.power[None, :], B * U, axis=0).reshape(P) / n_sin)
+        self.n_taps = n_taps
+
+    @staticmethod
+    def _pdp(tau: np.ndarray, rms: float) -> np.ndarray:
+        def spread(beta):
+            p = np.exp(-tau / beta)
+            p /= p.sum()
+            m1 = (p * tau).sum()
+            return math.sqrt(max((p * tau ** 2).sum() - m1 * m1, 0.0))
+        lo, hi = 1e-12, 1e-3
+        for _ in range(200):            # bisection: spread() grows with beta
+            mid = math.sqrt(lo * hi)
+            if spread(mid) < rms:
+                lo = mid
+            else:
+                hi = mid
+        p = np.exp(-tau / hi)
+        return p / p.sum()
+
+    def rms_delay_spread(self) -> float:
+        """Discrete RMS delay spread of the power-delay profile (s)."""
+        m1 = (self.power * self.tau).sum()
+        return math.sqrt((self.power * self.tau ** 2).sum() - m1 * m1)
+
+    def paths(self, t: int) -> np.ndarray:
+        """h[b][u][l] at slot t (complex128)."""
+        ts = t * self.slot_s
+        h = self.amp * np.exp(1j * (self.omega * ts + self.phi)).sum(axis=1)
+        return h.reshape(self.B, self.U, self.n_taps)
+
+    def slot(self, t: int) -> np.ndarray:
+        """A = H^H per group, complex64 [G][B][U]."""
+        H = np.einsum("bul,lg->gbu", self.paths(t), self.steer, optimize=True)
+        return np.conj(H).astype(np.complex64)
+
+
 # --------------------------------------------------------------------------
 # LLR layout (src/config/config.cc:84-135, src/operators/ldpc/config.h:61-72,
 # src/baseband/buffers.cc:97-103 with the 128-byte kCoalesce padding)
