@@ -23,16 +23,17 @@
 //        new[i][c] = (i == k or c == k ? 0 : a[i][c]) - a'_ik * a'_kc / a_kk,
 //      which after all pivots leaves -G^-1 (no pivoting: G is Hermitian
 //      positive definite and the pivots are exactly the squared Cholesky
-//      diagonals). For up to 16 users one wave does it in registers with
-//      cross-lane shuffles; for larger groups every thread keeps its entries
-//      in registers and the pivot row and column go through LDS (one barrier
-//      per pivot). A pivot not above U * 2^-23 * G_kk marks the
+//      diagonals). Entries stay in registers; the pivot row goes through
+//      LDS (its column is its conjugate), one barrier per pivot. A pivot
+//      not above U * 2^-23 * G_kk marks the
 //      group numerically singular: status k+1 and W = 0 (the reference never
 //      reads cuSOLVER's status and would emit Inf/NaN).
 //   3. W = G^-1 A = -(sweep result) A, M = users, N = 16 antennas, K = 4 users
 //      per step. A wave owns whole antenna columns (all user tiles), loads
 //      them before storing, so writing W over A (in place) is safe.
 #pragma once
+
+#include <type_traits>
 
 #include "eq_core.h"
 
@@ -41,13 +42,21 @@ namespace detail {
 
 constexpr float kPivotTol = 1.1920929e-7f;  // 2^-23, times U
 
+// f(integral_constant<int, I>) for I = B .. E-1, unrolled at compile time.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 template <int UT>
 struct ZfShared {
   static constexpr int UP = 16 * UT;
   static constexpr int LDG = UP + 1;  // padded row (complex elements)
-  static constexpr int NPUB = UT == 1 ? 1 : 2;  // multi-wave sweep: pivot row/column
   float2 G[UP * LDG];
-  float2 prow[NPUB][UP], pcol[NPUB][UP];
+  float2 prow[2][UP];
   float diag[UP];
 };
 
@@ -118,28 +127,36 @@ __device__ __forceinline__ float2 shfl2(float2 v, int src) {
   return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
-// One sweep-operator update of entry (i, c) for pivot k (see the file head).
-__device__ __forceinline__ float2 sweep_entry(float2 aic, float2 aik, float2 akc,
-                                              int i, int c, int k, float ip) {
-  const float2 base = (i == k || c == k) ? make_float2(0.f, 0.f) : aic;
-  const float2 x = (i == k) ? make_float2(-1.f, 0.f) : aik;
-  const float2 y = (c == k) ? make_float2(-1.f, 0.f) : akc;
-  const float2 p = cmul2(x, y);
-  return make_float2(base.x - p.x * ip, base.y - p.y * ip);
+// Reciprocal of a pivot: the hardware estimate and one Newton step.
+__device__ __forceinline__ float pivot_rcp(float p) {
+  float r = __builtin_amdgcn_rcpf(p);
+  const float e = __builtin_fmaf(-p, r, 1.0f);
+  return __builtin_fmaf(r, e, r);
 }
 
 // Overwrites the Gram matrix in sh.G (Hermitian, users padded to UP with an
 // identity block) with -G^-1 by U pivots of the sweep operator; returns 0 or
-// k+1 for the first pivot k at or below U * 2^-23 * G_kk (then sh.G holds a
-// partial result). All NTH threads of the workgroup call it after a barrier
-// that follows the writes of sh.G; it ends with a barrier.
+// k+1 for the first pivot k at or below U * 2^-23 * G_kk (sh.G then holds
+// unspecified values). All NTH threads of the workgroup call it after a
+// barrier that follows the writes of sh.G; it ends with a barrier.
+//
+// NS threads sweep (one wave for up to 16 users, else the workgroup): each
+// keeps EPT entries in registers, column c = tid % UP of rows
+// i = tid / UP + RS*j. The sweep keeps the matrix Hermitian, so column k is
+// the conjugate of row k and only row k is exchanged: its owners publish it
+// (double-buffered) right after updating it for pivot k-1, then one barrier,
+// then every thread issues its reads of row k and finishes the rest of its
+// pivot-(k-1) updates while they are in flight. The pivot loop is unrolled,
+// so the register index of every entry is static.
 template <int UT, int NTH>
 __device__ __forceinline__ int zf_sweep(ZfShared<UT>& sh, uint32_t U) {
   constexpr int UP = ZfShared<UT>::UP;
   constexpr int LDG = ZfShared<UT>::LDG;
+  constexpr int NS = UT == 1 ? 64 : NTH;
+  static_assert(NS <= NTH && (UP * UP) % NS == 0 && NS % UP == 0, "sweep layout");
+  constexpr int EPT = UP * UP / NS;
+  constexpr int RS = NS / UP;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int i16 = lane & 15, q4 = lane >> 4;
   float2* sG = sh.G;
   for (int i = tid; i < UP; i += NTH) {
     if (i >= (int)U) sG[i * LDG + i] = make_float2(1.f, 0.f);
@@ -149,86 +166,64 @@ __device__ __forceinline__ int zf_sweep(ZfShared<UT>& sh, uint32_t U) {
 
   const float tol = kPivotTol * (float)U;
   int fail = 0;
-  if constexpr (UT == 1) {
-    // One wave, in registers: lane holds column c = lane%16 of rows
-    // i = lane/16 + 4j (j = 0..3).
-    if (wave == 0) {
-      const int c = i16;
-      float2 a[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = sG[(q4 + 4 * j) * LDG + c];
-      for (int k = 0; k < (int)U; ++k) {
-        const int jk = k >> 2;
-        const float2 rowreg = jk == 0 ? a[0] : jk == 1 ? a[1] : jk == 2 ? a[2] : a[3];
-        const float2 akc = shfl2(rowreg, (k & 3) * 16 + c);   // a[k][c]
-        const float piv = __shfl(rowreg.x, (k & 3) * 16 + k, 64);
-        if (!(piv > tol * sh.diag[k]) || !isfinite(piv)) {
-          fail = k + 1;
-          break;
-        }
-        const float ip = 1.0f / piv;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float2 aik = shfl2(a[j], q4 * 16 + k);       // a[i][k], i = q4 + 4j
-          a[j] = sweep_entry(a[j], aik, akc, q4 + 4 * j, c, k, ip);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) sG[(q4 + 4 * j) * LDG + c] = a[j];
-      if (lane == 0) sh.diag[0] = __int_as_float(fail);
-    }
-    __syncthreads();
-    fail = __float_as_int(sh.diag[0]);
-  } else {
-    // Each thread keeps EPT entries in registers: column c = tid % UP of rows
-    // i = tid / UP + RS*j. Per pivot only row k and column k are needed from
-    // other threads; their owners publish them (double-buffered, so one
-    // barrier per pivot suffices) after updating them for the previous pivot.
-    static_assert((UP * UP) % NTH == 0 && NTH % UP == 0, "sweep layout");
-    constexpr int EPT = UP * UP / NTH;
-    constexpr int RS = NTH / UP;
+  if (tid < NS) {  // whole waves
     const int c = tid % UP, r0 = tid / UP;
     float2 a[EPT];
 #pragma unroll
     for (int j = 0; j < EPT; ++j) a[j] = sG[(r0 + RS * j) * LDG + c];
-    auto publish = [&](int k, int buf) {
+    // row k of the current matrix: entry c, entries r0 + RS*j, the pivot
+    float2 rc, ri[EPT];
+    float piv, dk;
+    auto read_row = [&](const float2* row, int k) {
+      rc = row[c];
 #pragma unroll
-      for (int j = 0; j < EPT; ++j) {
-        const int i = r0 + RS * j;
-        if (i == k) sh.prow[buf][c] = a[j];
-        if (c == k) sh.pcol[buf][i] = a[j];
-      }
+      for (int j = 0; j < EPT; ++j) ri[j] = row[r0 + RS * j];
+      piv = row[k].x;
+      dk = sh.diag[k];
     };
-    publish(0, 0);
-    __syncthreads();
-    for (uint32_t k = 0; k < U; ++k) {
-      const int buf = k & 1;
-      // every LDS read of this pivot in one round trip
-      const float piv = sh.prow[buf][k].x;
-      const float dk = sh.diag[k];
-      const float2 akc = sh.prow[buf][c];
-      float2 aik[EPT];
+    read_row(sG, 0);  // row 0 is still the Gram matrix's
+    static_for<0, UP>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if (k >= (int)U) return;  // uniform: pivots past U are not swept
+      const bool bad = !(piv > tol * dk) || !isfinite(piv);
+      fail = (fail == 0 && bad) ? k + 1 : fail;
+      const float ip = pivot_rcp(piv);
+      const bool ck = c == k;
+      const float2 yip = ck ? make_float2(-ip, 0.f) : make_float2(rc.x * ip, rc.y * ip);
+      float2 xk[EPT];
 #pragma unroll
-      for (int j = 0; j < EPT; ++j) aik[j] = sh.pcol[buf][r0 + RS * j];
-      if (!(piv > tol * dk) || !isfinite(piv)) {
-        fail = (int)k + 1;  // uniform: every thread read the same value
-        break;
+      for (int j = 0; j < EPT; ++j) xk[j] = make_float2(ri[j].x, -ri[j].y);  // a[i][k]
+      auto update = [&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const bool ik = (j == k / RS) && (r0 == k % RS);
+        const float2 x = ik ? make_float2(-1.f, 0.f) : xk[j];
+        const float2 base = (ik || ck) ? make_float2(0.f, 0.f) : a[j];
+        a[j] = make_float2(base.x - x.x * yip.x + x.y * yip.y,
+                           base.y - x.x * yip.y - x.y * yip.x);
+      };
+      constexpr int jn = (k + 1) / RS;  // register of row k+1 (if owned)
+      if constexpr (k + 1 < UP) {
+        update(std::integral_constant<int, jn>{});
+        if (r0 == (k + 1) % RS) sh.prow[(k + 1) & 1][c] = a[jn];
       }
-      const float ip = 1.0f / piv;
-#pragma unroll
-      for (int j = 0; j < EPT; ++j) {
-        const int i = r0 + RS * j;
-        a[j] = sweep_entry(a[j], aik[j], akc, i, c, (int)k, ip);
+      if constexpr (UT == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        __syncthreads();
       }
-      if (k + 1 < U) publish((int)k + 1, buf ^ 1);
-      __syncthreads();
-    }
-    if (!fail) {
+      if constexpr (k + 1 < UP) read_row(sh.prow[(k + 1) & 1], k + 1);
+      __builtin_amdgcn_sched_barrier(0);  // the reads go out before the updates
+      static_for<0, EPT>([&](auto jc) {
+        if constexpr (k + 1 >= UP || decltype(jc)::value != jn) update(jc);
+      });
+    });
 #pragma unroll
-      for (int j = 0; j < EPT; ++j) sG[(r0 + RS * j) * LDG + c] = a[j];
-    }
-    __syncthreads();
+    for (int j = 0; j < EPT; ++j) sG[(r0 + RS * j) * LDG + c] = a[j];
+    if (UT == 1 && tid == 0) sh.diag[0] = __int_as_float(fail);
   }
+  __syncthreads();
+  if constexpr (UT == 1) fail = __float_as_int(sh.diag[0]);
   return fail;
 }
 
