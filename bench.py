@@ -9,8 +9,10 @@ all 200 x 75 groups + equalize/demod of all 200 x 13 symbols, inputs resident
 in HBM. Three forms are timed: the fused FP32 kernel (ZF and equalize in one
 launch, W kept on chip; up to 16 users), the drop-in plain FP32 form (ZF
 kernel writing W, then the equalize kernel) and the mixed form (split-BF16
-matrix-core ZF and equalize). The headline is the fastest form whose EVM on
-slot 0 is within 0.1 dB of the FP64 oracle's. Synthetic data generated on the
+matrix-core ZF and equalize). Single-slot workloads (--workload c3_256x32)
+also time both forms slot-pipelined: ZF of step i+1 on a second stream while
+equalize of step i runs (W double-buffered, event-ordered). The headline is
+the fastest form whose EVM on slot 0 is within 0.1 dB of the FP64 oracle's. Synthetic data generated on the
 GPU with a seeded Philox generator.
 
 Prints one JSON line: value = detected symbols/s = U*N*S*slots / step time;
@@ -183,6 +185,37 @@ def main():
             evs[1].record(stream)
             evs[2].record(stream)
 
+    # Slot-pipelined forms (single-slot workloads): ZF of step i+1 runs on a
+    # second stream while equalize of step i runs, W double-buffered; every
+    # dependency is an event (ZF(i) -> EQ(i), EQ(i) -> ZF(i+2) reusing its W).
+    zs = torch.cuda.Stream()
+    wbuf = [w, torch.empty_like(w)]
+    sbuf = [split, torch.empty_like(split)]
+    pipe_state = {"i": 0, "eq_done": [None, None]}
+
+    def step_pipe(mixed, evs=None):
+        i = pipe_state["i"]
+        pipe_state["i"] = i + 1
+        k = i & 1
+        done = pipe_state["eq_done"][k]
+        if done is not None:
+            zs.wait_event(done)
+        zf_ev = torch.cuda.Event()
+        if mixed:
+            m.zf_split(csi, sbuf[k], B, U, slots * G, stream=zs)
+        else:
+            m.zf(csi, wbuf[k], B, U, slots * G, stream=zs)
+        zf_ev.record(zs)
+        stream.wait_event(zf_ev)
+        if mixed:
+            m.equalize_slots(sbuf[k], y, llr, form=m.FORM_MIXED, stream=stream,
+                             **dict(eq_kw, csi_slot_stride=G * U * Bp))
+        else:
+            m.equalize_slots(wbuf[k], y, llr, stream=stream, **eq_kw)
+        eq_ev = torch.cuda.Event()
+        eq_ev.record(stream)
+        pipe_state["eq_done"][k] = eq_ev
+
     if args.copy_calibration:
         src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
@@ -191,10 +224,21 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    def timed(step):
+    def timed(step, pipe=False):
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
+        if pipe:  # the timed region ends when both streams have drained
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record(stream)
+            zs.wait_stream(stream)
+            for i in range(args.steps):
+                step()
+            stream.wait_stream(zs)
+            t1.record(stream)
+            torch.cuda.synchronize()
+            return t0.elapsed_time(t1) / args.steps, None, None
         evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
                for _ in range(args.steps)]
         t0 = torch.cuda.Event(enable_timing=True)
@@ -217,6 +261,10 @@ def main():
     print(f"[bench] plain form timed ({args.steps} steps)", file=sys.stderr, flush=True)
     forms["mixed"] = timed(step_mixed)
     print(f"[bench] mixed form timed ({args.steps} steps)", file=sys.stderr, flush=True)
+    if slots == 1:
+        for name, mixed in (("plain_pipe", False), ("mixed_pipe", True)):
+            forms[name] = timed(lambda evs=None, mx=mixed: step_pipe(mx), pipe=True)
+            print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
 
     symbols = U * N * S * slots
     io_bytes = 8 * B * N * S * slots + 8 * U * B * G * slots + 2 * U * Q * N * S * slots
@@ -231,7 +279,7 @@ def main():
     x0 = x[0].cpu().numpy()
     evm = {}
     for name in forms:
-        if forms[name] is None:
+        if forms[name] is None or name.endswith("_pipe"):
             continue
         xh.zero_()
         if name == "fused":
@@ -245,6 +293,9 @@ def main():
                              stream=stream, **dict(kw1, csi_slot_stride=G * U * Bp))
         torch.cuda.synchronize()
         evm[name] = synth.evm_db(xh.cpu().numpy().reshape(S, U, N), x0)
+    for name in forms:
+        if name.endswith("_pipe"):  # same kernels and inputs as the sequential form
+            evm[name] = evm[name[:-5]]
     evm64 = fp64_evm(cfg, csi, y, x0)
     ok = {k: abs(v - evm64) < 0.1 for k, v in evm.items()}
 
@@ -255,7 +306,7 @@ def main():
     if head_form == "fused":
         dom_kernel, dom_ms, dom_prefix = ("detect_fused (ZF + equalize + demod)",
                                           forms["fused"][1], "detect_fused")
-    elif head_form == "plain":
+    elif head_form in ("plain", "plain_pipe"):
         dom_kernel, dom_ms, dom_prefix = "equalize_demod", forms["plain"][2], "equalize_"
     else:
         dom_kernel, dom_ms, dom_prefix = ("equalize_demod (split BF16)", forms["mixed"][2],
@@ -273,7 +324,10 @@ def main():
         ms, t1, t2 = forms[k]
         e = {"symbols_per_s": symbols / (ms * 1e-3), "ms_per_step": ms,
              "evm_db": evm[k], "evm_delta_db": evm[k] - evm64, "evm_ok": ok[k]}
-        if k != "fused":
+        if k.endswith("_pipe"):
+            e["schedule"] = ("ZF of step i+1 on a second stream during equalize of "
+                             "step i, W double-buffered")
+        elif k != "fused":
             e.update({"zf_ms": t1, "equalize_ms": t2,
                       "zf_tflops": zf_flops / (t1 * 1e-3) / 1e12,
                       "equalize_gbs": io_bytes / (t2 * 1e-3) / 1e9,
@@ -284,7 +338,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "symbols/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3 split, fp32 accumulate" if head_form == "mixed" else "fp32",
+        "dtype": "bf16x3 split, fp32 accumulate" if head_form.startswith("mixed") else "fp32",
         "data": "synthetic",
         "config": {"workload": args.workload, "antennas": B, "users": U,
                    "subcarriers": N, "sc_group": sg, "groups": G, "data_symbols": S,

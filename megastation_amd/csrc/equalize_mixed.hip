@@ -1,9 +1,9 @@
 // Equalize + demod of the mixed form: x = W y on the BF16 matrix cores with
 // split operands (prec_core.h; FP32 accumulation), W read as the split BF16
 // planes the mixed ZF kernel writes, y split on the fly. One wave per (slot,
-// group, 16-column tile, group of 16*MT users), four independent waves per
-// workgroup, as the streamed FP32 kernel (eq_core.h); demod and LLR
-// placement are the shared epilogue.
+// group, T 16-column tiles, group of 16*MT users), four independent waves per
+// workgroup, as the streamed FP32 kernel (eq_core.h); each W load serves T
+// column tiles. Demod and LLR placement are the shared epilogue.
 #include <cstdlib>
 
 #include "prec_core.h"
@@ -25,9 +25,9 @@ namespace {
 // their y is zeroed (the W planes hold zeros past B), so the ring needs no
 // early exit. NST > 0: the step count is known at compile time and the loop
 // is unrolled completely (exact waits; see eq_unit_stream).
-template <int MT, int R, int NST, int Q>
+template <int MT, int T, int R, int NST, int Q>
 __device__ __forceinline__ void eq_unit_mixed(const EqParams& p, const uint32_t* __restrict__ Ws,
-                                              uint32_t slot, uint32_t g, uint32_t t,
+                                              uint32_t slot, uint32_t g, uint32_t t0,
                                               uint32_t mg) {
   const int lane = threadIdx.x & 63;
   const int i16 = lane & 15, q4 = lane >> 4;
@@ -35,9 +35,14 @@ __device__ __forceinline__ void eq_unit_mixed(const EqParams& p, const uint32_t*
   const uint32_t Bp = split_row(B);
   const uint32_t k0 = g * p.sg;
   const uint32_t nk = min(p.sg, N - k0);
-  const uint32_t n = min(16 * t + i16, nk * p.S - 1);
-  const uint32_t sym = n / nk;
-  const float2* ycol = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
+  const float2* ycol[T];
+#pragma unroll
+  for (int c = 0; c < T; ++c) {
+    // columns past the group re-read its last one (their results are dropped)
+    const uint32_t n = min(16 * (t0 + c) + i16, nk * p.S - 1);
+    const uint32_t sym = n / nk;
+    ycol[c] = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
+  }
   const uint32_t* Wg = Ws + 2 * (slot * p.csi_slot + (uint64_t)g * U * Bp);
   const uint32_t* wrow[MT];
 #pragma unroll
@@ -46,10 +51,13 @@ __device__ __forceinline__ void eq_unit_mixed(const EqParams& p, const uint32_t*
   const uint64_t lo_off = (uint64_t)U * Bp;
   const uint32_t nst = NST > 0 ? (uint32_t)NST : Bp / 16;
 
-  auto load = [&](uint32_t st, float2 (&yd)[4], v4u (&wh)[MT], v4u (&wl)[MT]) {
+  auto load = [&](uint32_t st, float2 (&yd)[T][4], v4u (&wh)[MT], v4u (&wl)[MT]) {
     st = min(st, nst - 1);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) yd[a] = ycol[(uint64_t)min(16 * st + 4 * q4 + a, B - 1) * N];
+    for (int c = 0; c < T; ++c)
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        yd[c][a] = ycol[c][(uint64_t)min(16 * st + 4 * q4 + a, B - 1) * N];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
       wh[m] = *reinterpret_cast<const v4u*>(wrow[m] + 16 * st);
@@ -57,33 +65,39 @@ __device__ __forceinline__ void eq_unit_mixed(const EqParams& p, const uint32_t*
     }
   };
 
-  float2 yr[R][4];
+  float2 yr[R][T][4];
   v4u wh[R][MT], wl[R][MT];
 #pragma unroll
   for (int r = 0; r + 1 < R; ++r) load(r, yr[r], wh[r], wl[r]);
-  v4f are[MT], aim[MT];
+  v4f are[T][MT], aim[T][MT];
 #pragma unroll
-  for (int m = 0; m < MT; ++m) are[m] = aim[m] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int c = 0; c < T; ++c)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) are[c][m] = aim[c][m] = v4f{0.f, 0.f, 0.f, 0.f};
 
   auto step = [&](uint32_t st, int r) {
     load(st + R - 1, yr[(r + R - 1) % R], wh[(r + R - 1) % R], wl[(r + R - 1) % R]);
     __builtin_amdgcn_sched_barrier(0);
-    v4u b1h, b1l, b2h, b2l;
+    v4u b1h[T], b1l[T], b2h[T], b2l[T];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const bool ok = 16 * st + 4 * q4 + a < B;
-      const SplitC sp = split_c(ok ? yr[r][a] : make_float2(0.f, 0.f));
-      b1h[a] = sp.hi ^ 0x80000000u;
-      b1l[a] = sp.lo ^ 0x80000000u;
-      b2h[a] = __builtin_rotateleft32(sp.hi, 16);
-      b2l[a] = __builtin_rotateleft32(sp.lo, 16);
-    }
+    for (int c = 0; c < T; ++c)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const bool ok = 16 * st + 4 * q4 + a < B;
+        const SplitC sp = split_c(ok ? yr[r][c][a] : make_float2(0.f, 0.f));
+        b1h[c][a] = sp.hi ^ 0x80000000u;
+        b1l[c][a] = sp.lo ^ 0x80000000u;
+        b2h[c][a] = __builtin_rotateleft32(sp.hi, 16);
+        b2l[c][a] = __builtin_rotateleft32(sp.lo, 16);
+      }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      are[m] = mfma_x3(wh[r][m], wl[r][m], b1h, b1l, are[m]);
-      aim[m] = mfma_x3(wh[r][m], wl[r][m], b2h, b2l, aim[m]);
-    }
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int c = 0; c < T; ++c) {
+        are[c][m] = mfma_x3(wh[r][m], wl[r][m], b1h[c], b1l[c], are[c][m]);
+        aim[c][m] = mfma_x3(wh[r][m], wl[r][m], b2h[c], b2l[c], aim[c][m]);
+      }
   };
   if constexpr (NST > 0) {
 #pragma unroll
@@ -94,16 +108,22 @@ __device__ __forceinline__ void eq_unit_mixed(const EqParams& p, const uint32_t*
       for (int r = 0; r < R; ++r) step(s0 + r, r);
     }
   }
-  const EqCol col = eq_col<Q>(p, slot, g, t);
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const uint32_t u0 = 16 * (mg * MT + m) + 4 * q4;
-    eq_store<Q>(p, col, u0, v4f{are[m][0], aim[m][0], are[m][1], aim[m][1]});
-    eq_store<Q>(p, col, u0 + 2, v4f{are[m][2], aim[m][2], are[m][3], aim[m][3]});
+  for (int c = 0; c < T; ++c) {
+    if (16 * (t0 + c) >= nk * p.S) break;  // uniform: past the group
+    const EqCol col = eq_col<Q>(p, slot, g, t0 + c);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const uint32_t u0 = 16 * (mg * MT + m) + 4 * q4;
+      eq_store<Q>(p, col, u0, v4f{are[c][m][0], aim[c][m][0], are[c][m][1], aim[c][m][1]});
+      eq_store<Q>(p, col, u0 + 2, v4f{are[c][m][2], aim[c][m][2], are[c][m][3], aim[c][m][3]});
+    }
   }
 }
 
-template <int MT, int R, int NST, int Q>
+// One wave per (slot, group, T column tiles, user group); p.col_tiles counts
+// units of T tiles here.
+template <int MT, int T, int R, int NST, int Q>
 __global__ __launch_bounds__(256) void equalize_mixed(const EqParams p, const uint32_t* Ws,
                                                       uint32_t units) {
   const uint32_t unit = xcd_linear(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
@@ -112,19 +132,19 @@ __global__ __launch_bounds__(256) void equalize_mixed(const EqParams p, const ui
   const uint32_t t = (unit / p.nmg) % p.col_tiles;
   const uint32_t rest = unit / p.nmg / p.col_tiles;
   const uint32_t g = rest % p.G, slot = rest / p.G;
-  if (16 * t >= min(p.sg, p.N - g * p.sg) * p.S) return;  // past a partial group
-  eq_unit_mixed<MT, R, NST, Q>(p, Ws, slot, g, t, mg);
+  if (16 * T * t >= min(p.sg, p.N - g * p.sg) * p.S) return;  // past a partial group
+  eq_unit_mixed<MT, T, R, NST, Q>(p, Ws, slot, g, T * t, mg);
 }
 
-template <int MT, int R, int NST>
+template <int MT, int T, int R, int NST>
 hipError_t launch_q(const EqParams& p, const uint32_t* wp, uint32_t units, uint32_t Q,
                     hipStream_t st) {
   const dim3 grid((units + 3) / 4), block(256);
   switch (Q) {
-    case 2: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 2>), grid, block, 0, st, p, wp, units); break;
-    case 4: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 4>), grid, block, 0, st, p, wp, units); break;
-    case 6: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 6>), grid, block, 0, st, p, wp, units); break;
-    case 8: hipLaunchKernelGGL((equalize_mixed<MT, R, NST, 8>), grid, block, 0, st, p, wp, units); break;
+    case 2: hipLaunchKernelGGL((equalize_mixed<MT, T, R, NST, 2>), grid, block, 0, st, p, wp, units); break;
+    case 4: hipLaunchKernelGGL((equalize_mixed<MT, T, R, NST, 4>), grid, block, 0, st, p, wp, units); break;
+    case 6: hipLaunchKernelGGL((equalize_mixed<MT, T, R, NST, 6>), grid, block, 0, st, p, wp, units); break;
+    case 8: hipLaunchKernelGGL((equalize_mixed<MT, T, R, NST, 8>), grid, block, 0, st, p, wp, units); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -132,15 +152,15 @@ hipError_t launch_q(const EqParams& p, const uint32_t* wp, uint32_t units, uint3
 
 // Ring depth 4 (one 16-user tile per wave) or 3 (two tiles); B up to 256 in
 // whole steps gets a fully unrolled step loop.
-template <int MT>
+template <int MT, int T>
 hipError_t launch_mt(const EqParams& p, const uint32_t* wp, uint32_t units, uint32_t Q,
                      hipStream_t st) {
-  constexpr int R = MT == 2 ? 3 : 4;
+  constexpr int R = MT * T >= 2 ? 3 : 4;
   switch (split_row(p.B)) {
-    case 64: return launch_q<MT, R, 4>(p, wp, units, Q, st);
-    case 128: return launch_q<MT, R, 8>(p, wp, units, Q, st);
-    case 256: return launch_q<MT, R, 16>(p, wp, units, Q, st);
-    default: return launch_q<MT, R, 0>(p, wp, units, Q, st);
+    case 64: return launch_q<MT, T, R, 4>(p, wp, units, Q, st);
+    case 128: return launch_q<MT, T, R, 8>(p, wp, units, Q, st);
+    case 256: return launch_q<MT, T, R, 16>(p, wp, units, Q, st);
+    default: return launch_q<MT, T, R, 0>(p, wp, units, Q, st);
   }
 }
 
@@ -165,18 +185,26 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStrea
   uint32_t MT = a.U <= 16 ? 1 : 2;
   const int smt = env_int("MEGA_EQ_XMT");
   if (smt == 1 || smt == 2) MT = (uint32_t)smt;
+  // Column tiles per wave: 2 (one W stream for 32 columns) when a group has
+  // more than one tile, else 1; MEGA_EQ_XT=1|2 overrides.
+  uint32_t T = p.col_tiles > 1 ? 2 : 1;
+  const int xt = env_int("MEGA_EQ_XT");
+  if (xt == 1 || xt == 2) T = (uint32_t)xt;
   p.nmg = (a.U + 16 * MT - 1) / (16 * MT);
   p.nwk = 1;
   p.chunks = 1;
   p.tiles_per_wg = p.col_tiles;
+  p.col_tiles = (p.col_tiles + T - 1) / T;  // units of T tiles
   const uint64_t units = (uint64_t)a.slots * a.G * p.col_tiles * p.nmg;
   if (units > 0xffffffffull - 3) {
     *why = "grid too large";
     return hipErrorInvalidValue;
   }
   const uint32_t* w = static_cast<const uint32_t*>(wp);
-  return MT == 1 ? launch_mt<1>(p, w, (uint32_t)units, a.Q, st)
-                 : launch_mt<2>(p, w, (uint32_t)units, a.Q, st);
+  const uint32_t n = (uint32_t)units;
+  if (T == 2)
+    return MT == 1 ? launch_mt<1, 2>(p, w, n, a.Q, st) : launch_mt<2, 2>(p, w, n, a.Q, st);
+  return MT == 1 ? launch_mt<1, 1>(p, w, n, a.Q, st) : launch_mt<2, 1>(p, w, n, a.Q, st);
 }
 
 }  // namespace detail

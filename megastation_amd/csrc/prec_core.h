@@ -55,9 +55,17 @@ __device__ __forceinline__ SplitC split_c(float2 z) {
 }
 
 // A-operand word of an interleaved output row: h = 0 (re row) -> (re, -im),
-// h = 1 (im row) -> (im, re).
-__device__ __forceinline__ uint32_t a_word(uint32_t w, bool h) {
-  return h ? __builtin_rotateleft32(w, 16) : (w ^ 0x80000000u);
+// h = 1 (im row) -> (im, re). Branch-free for a lane-varying h: a rotate by
+// 16*h then a sign flip of the high half when h = 0 (ARow holds both
+// per-lane constants).
+struct ARow {
+  uint32_t rot, flip;
+  __device__ __forceinline__ explicit ARow(bool h)
+      : rot(h ? 16u : 0u), flip(h ? 0u : 0x80000000u) {}
+};
+
+__device__ __forceinline__ uint32_t a_word(uint32_t w, ARow r) {
+  return __builtin_amdgcn_alignbit(w, w, r.rot) ^ r.flip;
 }
 
 // acc += X Z over one K step of 16 complex indices, both split.

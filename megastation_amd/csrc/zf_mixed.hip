@@ -37,22 +37,37 @@ struct ZfMixShared {
   ZfShared<UT> sw;     // G, then -G^-1 (sweep); later X (refined inverse)
   float2 G0[UP * LDG];  // copy of G for the residual; later R; later split X
   float red[2];
+  float part[kMixThreads];  // column-norm shares
 };
 
-__device__ __forceinline__ float cabs2(float2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
 
-// max over columns c < U of sum_i |M[i][c]|, by wave 0 (U <= 64); result in
-// *out after the caller's next barrier.
-template <int LDG>
-__device__ __forceinline__ void col_norm1(const float2* M, uint32_t U, float* out) {
-  if (threadIdx.x >= 64) return;
-  const uint32_t c = threadIdx.x;
+// max over columns c < U of sum_i |M[i][c]| (|.| from the hardware square
+// root; the norm only feeds the condition estimate), result in *out after the
+// caller's next barrier. Every thread sums its share of a column into part,
+// then wave 0 adds the shares and takes the maximum. All threads call it.
+template <int UP, int LDG>
+__device__ __forceinline__ void col_norm1(const float2* M, uint32_t U, float* part, float* out) {
+  constexpr int RS = kMixThreads / UP, EPT = UP / RS;
+  const int tid = threadIdx.x, c = tid % UP, r0 = tid / UP;
   float s = 0.f;
-  if (c < U)
-    for (uint32_t i = 0; i < U; ++i) s += cabs2(M[i * LDG + c]);
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s = fmaxf(s, __shfl_xor(s, off, 64));
-  if (c == 0) *out = s;
+  for (int j = 0; j < EPT; ++j) {
+    const int i = r0 + RS * j;
+    const float2 v = M[i * LDG + c];
+    s += i < (int)U ? __builtin_amdgcn_sqrtf(v.x * v.x + v.y * v.y) : 0.f;
+  }
+  part[tid] = s;
+  __syncthreads();
+  if (tid < 64) {
+    const int cc = tid < UP ? tid : UP - 1;
+    float t = 0.f;
+#pragma unroll
+    for (int r = 0; r < RS; ++r) t += part[r * UP + cc];
+    if (tid >= (int)U) t = 0.f;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t = fmaxf(t, __shfl_xor(t, off, 64));
+    if (tid == 0) *out = t;
+  }
 }
 
 // FP64 sweep on M (UP x LDG, Hermitian, padded with identity) -> -M^-1, one
@@ -137,8 +152,158 @@ __device__ __forceinline__ void store_pad(uint32_t* ws, uint32_t U, uint32_t Bp,
   }
 }
 
+// ---- Gram on split BF16 products ------------------------------------------
+// Antenna chunks of kStageKC go through LDS once per group as split words:
+// every thread loads its share of kStageBatch chunks at once (coalesced, the
+// whole batch in flight), then per chunk splits it once and writes the hi
+// and lo planes [user][antenna] (rows of kStageKCP words, 16-byte aligned),
+// double-buffered with one barrier per chunk. Output tiles are 8 users (rows, re/im
+// interleaved) x 16 users (columns); only tiles reaching the lower triangle
+// are computed (the caller mirrors), spread over the four waves.
+constexpr int kStageKC = 64;
+constexpr int kStageKCP = kStageKC + 4;
+constexpr int kStageBatch = 4;
+
 template <int UT>
-__global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict__ in,
+__host__ __device__ constexpr size_t gram_stage_bytes() {
+  return (size_t)2 * 2 * 16 * UT * kStageKCP * sizeof(uint32_t);
+}
+
+template <int UT>
+struct LowerTiles {
+  static constexpr int RT = 2 * UT, CT = UT;
+  static constexpr bool lower(int rt, int ct) { return 8 * rt + 7 >= 16 * ct; }
+  static constexpr int count() {
+    int n = 0;
+    for (int rt = 0; rt < RT; ++rt)
+      for (int ct = 0; ct < CT; ++ct) n += lower(rt, ct) ? 1 : 0;
+    return n;
+  }
+  static constexpr int N = count();
+  // (rt, ct) of lower tile t, packed as rt * CT + ct
+  static constexpr int at(int t) {
+    for (int rt = 0; rt < RT; ++rt)
+      for (int ct = 0; ct < CT; ++ct)
+        if (lower(rt, ct) && t-- == 0) return rt * CT + ct;
+    return 0;
+  }
+};
+
+template <int UT>
+__device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_t U, uint32_t B,
+                                           uint32_t* stage, float2* sG) {
+  constexpr int UP = 16 * UT, LDG = UP + 1, NTH = kMixThreads, NW = NTH / 64;
+  constexpr int PER = UP * kStageKC / NTH;  // staged elements per thread
+  using T = LowerTiles<UT>;
+  constexpr int TW = (T::N + NW - 1) / NW;  // tiles per wave (at most)
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const ARow h(i16 & 1);
+  const uint32_t nch = (B + kStageKC - 1) / kStageKC;
+  auto plane = [&](int buf, int lo) { return stage + (2 * buf + lo) * UP * kStageKCP; };
+
+  // all chunks of a batch of up to kStageBatch are loaded at once, so the
+  // group's A is in flight together (one HBM round trip per batch)
+  constexpr int NB = UT >= 4 ? kStageBatch / 2 : kStageBatch;
+  float2 pf[NB][PER];
+  auto fetch = [&](uint32_t ch, float2 (&d)[PER]) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t e = tid + NTH * j;
+      const uint32_t u = e % UP, b = ch * kStageKC + e / UP;
+      d[j] = A[min(u, U - 1) + (uint64_t)U * min(b, B - 1)];
+    }
+  };
+  auto put = [&](uint32_t ch, const float2 (&d)[PER], int buf) {
+    uint32_t* ph = plane(buf, 0);
+    uint32_t* pl = plane(buf, 1);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t e = tid + NTH * j;
+      const uint32_t u = e % UP, bl = e / UP;
+      const bool ok = u < U && ch * kStageKC + bl < B;
+      const SplitC sp = split_c(ok ? d[j] : make_float2(0.f, 0.f));
+      ph[u * kStageKCP + bl] = sp.hi;
+      pl[u * kStageKCP + bl] = sp.lo;
+    }
+  };
+
+  v4f acc[TW], acc2[TW], acc3[TW];
+  int trow[TW], tcol[TW];  // first row / column of the wave's tiles (-1: none)
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+#pragma unroll
+  for (int t = 0; t < TW; ++t) {
+    acc[t] = acc2[t] = acc3[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int idx = wv + NW * t;
+    const int rc = idx < T::N ? T::at(idx) : -1;
+    trow[t] = rc < 0 ? -1 : 8 * (rc / T::CT);
+    tcol[t] = rc < 0 ? -1 : 16 * (rc % T::CT);
+  }
+  // Per 16-antenna step, the operands of all the wave's tiles are read
+  // first, then the three products of every tile go to separate
+  // accumulators (no dependent chains; summed at the end).
+  auto products = [&](int buf) {
+    const uint32_t* ph = plane(buf, 0);
+    const uint32_t* pl = plane(buf, 1);
+#pragma unroll
+    for (int st = 0; st < kStageKC / 16; ++st) {
+      const int k0 = 16 * st + 4 * q4;
+      v4u ah[TW], al[TW], bh[TW], bl[TW];
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+        const uint32_t u = max(trow[t], 0) + (i16 >> 1), v = max(tcol[t], 0) + i16;
+        const v4u uh = *reinterpret_cast<const v4u*>(ph + u * kStageKCP + k0);
+        const v4u ul = *reinterpret_cast<const v4u*>(pl + u * kStageKCP + k0);
+        const v4u vh = *reinterpret_cast<const v4u*>(ph + v * kStageKCP + k0);
+        const v4u vl = *reinterpret_cast<const v4u*>(pl + v * kStageKCP + k0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          ah[t][a] = a_word(uh[a], h);
+          al[t][a] = a_word(ul[a], h);
+          bh[t][a] = vh[a] ^ 0x80000000u;  // conj(A[v][b])
+          bl[t][a] = vl[a] ^ 0x80000000u;
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TW; ++t) {
+        if (trow[t] < 0) continue;  // uniform per wave
+        acc[t] = mfma_bf16(ah[t], bh[t], acc[t]);
+        acc2[t] = mfma_bf16(al[t], bh[t], acc2[t]);
+        acc3[t] = mfma_bf16(ah[t], bl[t], acc3[t]);
+      }
+    }
+  };
+  // chunks ch0 .. ch0+NB-1 per batch; chunk c of the batch uses buffer c & 1
+  for (uint32_t ch0 = 0; ch0 < nch; ch0 += NB) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) fetch(min(ch0 + c, nch - 1), pf[c]);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      if (ch0 + c >= nch) break;  // uniform
+      put(ch0 + c, pf[c], c & 1);
+      __syncthreads();
+      products(c & 1);
+    }
+    __syncthreads();  // the last buffers are read before the next batch's writes
+  }
+  static_for<0, TW>([&](auto tc) {
+    constexpr int t = decltype(tc)::value;
+    if (trow[t] < 0) return;
+    acc[t] += acc2[t] + acc3[t];  // hi*hi + (lo*hi + hi*lo)
+    const uint32_t col = tcol[t] + i16;
+#pragma unroll
+    for (int r2 = 0; r2 < 2; ++r2) {
+      const uint32_t row = trow[t] + 2 * q4 + r2;
+      const bool ok = row < U && col < U;
+      sG[row * LDG + col] =
+          ok ? make_float2(acc[t][2 * r2], acc[t][2 * r2 + 1]) : make_float2(0.f, 0.f);
+    }
+  });
+}
+
+template <int UT>
+__global__ __launch_bounds__(kMixThreads, UT >= 4 ? 1 : 2) void zf_mixed(const float2* __restrict__ in,
                                                         float2* w_out, uint32_t* ws_out,
                                                         int32_t* info, int32_t* route,
                                                         uint32_t U, uint32_t B) {
@@ -147,71 +312,21 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
   constexpr int NTH = kMixThreads;
   constexpr int NW = NTH / 64;
   __shared__ ZfMixShared<UT> sh;
-  extern __shared__ double2 sM64[];  // FP64 route: UP x LDG, then prow/pcol, diag
+  // Gram staging (gram_split), later the FP64 route: UP x LDG, prow/pcol, diag
+  extern __shared__ double2 sM64[];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int i16 = lane & 15, q4 = lane >> 4;
-  const bool h = i16 & 1;
+  const ARow h(i16 & 1);
   const uint32_t g = blockIdx.x;
   const float2* A = in + (uint64_t)g * U * B;
   float2* w = w_out ? w_out + (uint64_t)g * U * B : nullptr;
   const uint32_t Bp = split_row(B);
   uint32_t* ws = ws_out ? ws_out + (uint64_t)g * 2 * U * Bp : nullptr;
   float2* sG = sh.sw.G;
-  const uint32_t nst = (B + 15) / 16;
 
-  // ---- 1. Gram on split BF16 products: tiles of 8 (rows) x 16 (cols) users --
-  // Antenna steps of 16 stream through a ring of RG register sets (RG-1
-  // steps in flight); loads past the last step re-read it (clamped) and are
-  // masked to zero like antennas past B.
-  {
-    constexpr int RT = 2 * UT, CT = UT, RG = 4;
-    for (int tile = wave; tile < RT * CT; tile += NW) {
-      const int rt = tile / CT, ct = tile % CT;
-      const uint32_t u = min(8u * rt + (i16 >> 1), U - 1);
-      const uint32_t v = min(16u * ct + i16, U - 1);
-      float2 zu[RG][4], zv[RG][4];
-      auto load = [&](uint32_t st, float2 (&du)[4], float2 (&dv)[4]) {
-        st = min(st, nst - 1);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const uint32_t b = min(16 * st + 4 * q4 + a, B - 1);
-          du[a] = A[u + (uint64_t)U * b];
-          dv[a] = A[v + (uint64_t)U * b];
-        }
-      };
-#pragma unroll
-      for (int r = 0; r + 1 < RG; ++r) load(r, zu[r], zv[r]);
-      v4f acc = {0.f, 0.f, 0.f, 0.f};
-      for (uint32_t s0 = 0; s0 < nst; s0 += RG) {
-#pragma unroll
-        for (int r = 0; r < RG; ++r) {
-          const uint32_t st = s0 + r;
-          load(st + RG - 1, zu[(r + RG - 1) % RG], zv[(r + RG - 1) % RG]);
-          __builtin_amdgcn_sched_barrier(0);
-          v4u ah, al, bh, bl;
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            const bool ok = 16 * st + 4 * q4 + a < B;
-            const SplitC su = split_c(ok ? zu[r][a] : make_float2(0.f, 0.f));
-            const SplitC sv = split_c(zv[r][a]);
-            ah[a] = a_word(su.hi, h);
-            al[a] = a_word(su.lo, h);
-            bh[a] = sv.hi ^ 0x80000000u;  // conj(A[v][b])
-            bl[a] = sv.lo ^ 0x80000000u;
-          }
-          acc = mfma_x3(ah, al, bh, bl, acc);
-        }
-      }
-      const uint32_t col = 16 * ct + i16;
-#pragma unroll
-      for (int r2 = 0; r2 < 2; ++r2) {
-        const uint32_t row = 8 * rt + 2 * q4 + r2;
-        const bool ok = row < U && col < U;
-        sG[row * LDG + col] = ok ? make_float2(acc[2 * r2], acc[2 * r2 + 1]) : make_float2(0.f, 0.f);
-      }
-    }
-  }
+  // ---- 1. Gram on split BF16 products --------------------------------------
+  gram_split<UT>(A, U, B, reinterpret_cast<uint32_t*>(sM64), sG);
   __syncthreads();
   // exact Hermitian symmetry (lower triangle mirrored) and the copy for R
   for (int e = tid; e < UP * UP; e += NTH) {
@@ -226,13 +341,13 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
     sh.G0[i * LDG + c] = val;
   }
   __syncthreads();
-  col_norm1<LDG>(sh.G0, U, &sh.red[0]);
+  col_norm1<UP, LDG>(sh.G0, U, sh.part, &sh.red[0]);
 
   // ---- 2. FP32 sweep and condition number ------------------------------------
   int fail = zf_sweep<UT, NTH>(sh.sw, U);  // starts with the barrier it needs
   bool fp64 = fail != 0;
   if (!fp64) {
-    col_norm1<LDG>(sG, U, &sh.red[1]);
+    col_norm1<UP, LDG>(sG, U, sh.part, &sh.red[1]);
     __syncthreads();
     const float kappa = sh.red[0] * sh.red[1];
     fp64 = !(kappa <= kCondMax);
@@ -248,7 +363,8 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
     for (int j = 0; j < EPT; ++j) {
       const int i = r0 + RS * j;
       double sr = (i == c) ? 1.0 : 0.0, si = 0.0;
-      for (uint32_t k = 0; k < U; ++k) {
+#pragma unroll
+      for (int k = 0; k < UP; ++k) {  // G0 is zero past U
         const float2 gk = sh.G0[i * LDG + k];
         const float2 xk = sG[k * LDG + c];  // -X0
         sr += (double)gk.x * xk.x - (double)gk.y * xk.y;
@@ -266,7 +382,8 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
       const int i = r0 + RS * j;
       const float2 x0 = sG[i * LDG + c];
       float xr = -x0.x, xi = -x0.y;
-      for (uint32_t k = 0; k < U; ++k) {
+#pragma unroll
+      for (int k = 0; k < UP; ++k) {  // -X0 is zero past U off its diagonal, R past U
         const float2 xk = sG[i * LDG + k];
         const float2 rk = sh.G0[k * LDG + c];
         xr -= xk.x * rk.x - xk.y * rk.y;
@@ -285,26 +402,37 @@ __global__ __launch_bounds__(kMixThreads) void zf_mixed(const float2* __restrict
     __syncthreads();
 
     // ---- 4. W = X A on split BF16 products ----------------------------------
-    // A tiles of 16 antennas x all users, double-buffered across the wave's
-    // tiles (the next tile's loads in flight during this tile's products).
+    // A tiles of 16 antennas x all users; a wave's tiles are loaded in
+    // batches of NAB, all in flight together, then multiplied and stored.
     const uint32_t ntiles = (B + 15) / 16;
-    float2 zb[2][UT][4];
+    constexpr int NAB = 1;
+    const bool vec = U == UP;  // whole rows of four users: 16-byte loads
+    float2 zb[NAB][UT][4];
     auto load_a = [&](uint32_t nt, float2 (&d)[UT][4]) {
       const uint32_t bc = min(16 * min(nt, ntiles - 1) + i16, B - 1);
 #pragma unroll
-      for (int ks = 0; ks < UT; ++ks)
+      for (int ks = 0; ks < UT; ++ks) {
+        if (vec) {
+          const float4* p4 = reinterpret_cast<const float4*>(A + 16u * ks + 4 * q4 + (uint64_t)U * bc);
+          const float4 x0 = p4[0], x1 = p4[1];
+          d[ks][0] = make_float2(x0.x, x0.y);
+          d[ks][1] = make_float2(x0.z, x0.w);
+          d[ks][2] = make_float2(x1.x, x1.y);
+          d[ks][3] = make_float2(x1.z, x1.w);
+        } else {
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-          d[ks][a] = A[min(16u * ks + 4 * q4 + a, U - 1) + (uint64_t)U * bc];
+          for (int a = 0; a < 4; ++a)
+            d[ks][a] = A[min(16u * ks + 4 * q4 + a, U - 1) + (uint64_t)U * bc];
+        }
+      }
     };
-    load_a(wave, zb[0]);
-    for (uint32_t n0 = wave; n0 < ntiles; n0 += 2 * NW) {
+    for (uint32_t n0 = wave; n0 < ntiles; n0 += NAB * NW) {
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
+      for (int r = 0; r < NAB; ++r) load_a(n0 + r * NW, zb[r]);
+#pragma unroll
+      for (int r = 0; r < NAB; ++r) {
         const uint32_t nt = n0 + r * NW;
-        if (nt >= ntiles) break;
-        load_a(nt + NW, zb[r ^ 1]);
-        __builtin_amdgcn_sched_barrier(0);
+        if (nt >= ntiles) break;  // uniform per wave
         const uint32_t b = 16 * nt + i16;
         v4u bh[UT], bl[UT];
 #pragma unroll
@@ -401,8 +529,9 @@ hipError_t launch_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
                      float2* w_out, uint32_t* wp, int32_t* info, int32_t* route,
                      hipStream_t st) {
   constexpr int UP = 16 * UT;
-  const size_t lds = (size_t)UP * (UP + 1) * sizeof(double2) + 4 * UP * sizeof(double2) +
+  const size_t f64 = (size_t)UP * (UP + 1) * sizeof(double2) + 4 * UP * sizeof(double2) +
                      UP * sizeof(double);
+  const size_t lds = f64 > gram_stage_bytes<UT>() ? f64 : gram_stage_bytes<UT>();
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zf_mixed<UT>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
