@@ -8,11 +8,14 @@
 // B(k,j) = B[k*sbk + j*sbj] (any transpose through the strides), C row-major
 // FP32 (ldc) and/or BF16 with an epilogue: + bias[j], then GELU (tanh form)
 // or identity. 64 x 64 tiles per 256-thread workgroup, K in steps of 32
-// staged through LDS as BF16; each wave owns a 32 x 32 quarter (2 x 2
-// v_mfma_f32_16x16x32_bf16 tiles).
+// staged through LDS as BF16 (the next step's vectors loaded during this
+// step's products); each wave owns a 32 x 32 quarter (2 x 2
+// v_mfma_f32_16x16x32_bf16 tiles). Small M x N grids split K over more
+// workgroups and add the partial sums in a second kernel.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "mlp.h"
 
@@ -37,59 +40,81 @@ __device__ __forceinline__ float gelu_grad(float z) {
 
 constexpr int TM = 64, TN = 64, TK = 32;
 
+// Bias, activation and the FP32 / BF16 stores of one output element.
+__device__ __forceinline__ void gemm_out(float acc, uint32_t i, uint32_t j,
+                                         const float* __restrict__ bias, int act, float* Cf,
+                                         __bf16* Cb, uint32_t ldc, float beta) {
+  const float z = acc + (bias ? bias[j] : 0.f);
+  if (Cf) {
+    const uint64_t o = (uint64_t)i * ldc + j;
+    Cf[o] = beta != 0.f ? z + beta * Cf[o] : z;  // FP32 output (pre-activation)
+  }
+  if (Cb) Cb[(uint64_t)i * ldc + j] = (__bf16)(act == 1 ? gelu(z) : z);
+}
+
+// blockIdx.z = K split: K range [z * kchunk, (z+1) * kchunk). With part, the
+// split's raw sums go to part[z][M][N] (gemm_reduce adds the splits in order
+// and applies the epilogue); else the epilogue is applied here.
 __global__ __launch_bounds__(256) void gemm_bf16(const __bf16* __restrict__ A, int64_t sai,
                                                  int64_t sak, const __bf16* __restrict__ B,
                                                  int64_t sbk, int64_t sbj, uint32_t M,
-                                                 uint32_t N, uint32_t K,
+                                                 uint32_t N, uint32_t K, uint32_t kchunk,
                                                  const float* __restrict__ bias, int act,
                                                  float* __restrict__ Cf, __bf16* __restrict__ Cb,
-                                                 uint32_t ldc, float beta) {
+                                                 uint32_t ldc, float beta, float* part) {
   __shared__ __bf16 sA[TM][TK + 8];
   __shared__ __bf16 sB[TN][TK + 8];  // transposed: sB[j][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i16 = lane & 15, q4 = lane >> 4;
   const uint32_t m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const uint32_t kb0 = blockIdx.z * kchunk, kb1 = min(K, kb0 + kchunk);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   v4f acc[2][2] = {};
   const bool a_kfast = sak == 1, b_jfast = sbj == 1;
   // 16-byte staging needs the rows 16-byte aligned
   const bool a_vec = a_kfast && (sai % 8) == 0 && (reinterpret_cast<uintptr_t>(A) % 16) == 0;
   const bool b_vec = b_jfast && (sbk % 8) == 0 && (reinterpret_cast<uintptr_t>(B) % 16) == 0;
-  for (uint32_t k0 = 0; k0 < K; k0 += TK) {
-    // stage: 64 x 32 of A and 32 x 64 of B, 8 elements per thread each.
-    // Whole 16-byte vectors when the unit-stride dimension is the tile's
-    // contiguous one and the tile lies inside the matrix; else element by
-    // element with consecutive threads on the unit-stride dimension.
-    if (a_vec && m0 + TM <= M && k0 + TK <= K) {
-      const int ii = tid / 4, kk = (tid % 4) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(A + (uint64_t)(m0 + ii) * sai + k0 + kk);
-      *reinterpret_cast<uint4*>(&sA[ii][kk]) = v;
+  // Whole 16-byte vectors when the unit-stride dimension is the tile's
+  // contiguous one and the tile lies inside the matrix (loaded one K step
+  // ahead into registers); else element by element with consecutive threads
+  // on the unit-stride dimension.
+  const int ai = tid / 4, ak = (tid % 4) * 8;   // A vector slot
+  const int bk = tid / 8, bj = (tid % 8) * 8;   // B vector slot
+  auto a_fast = [&](uint32_t k0) { return a_vec && m0 + TM <= M && k0 + TK <= kb1; };
+  auto b_fast = [&](uint32_t k0) { return b_vec && n0 + TN <= N && k0 + TK <= kb1; };
+  uint4 ra = {}, rb = {};
+  auto fetch = [&](uint32_t k0) {
+    if (a_fast(k0)) ra = *reinterpret_cast<const uint4*>(A + (uint64_t)(m0 + ai) * sai + k0 + ak);
+    if (b_fast(k0)) rb = *reinterpret_cast<const uint4*>(B + (uint64_t)(k0 + bk) * sbk + n0 + bj);
+  };
+  fetch(kb0);
+  for (uint32_t k0 = kb0; k0 < kb1; k0 += TK) {
+    if (a_fast(k0)) {
+      *reinterpret_cast<uint4*>(&sA[ai][ak]) = ra;
     } else {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int e = tid + 256 * r;  // 0..2047
         const int ii = a_kfast ? e / TK : e % TM, kk = a_kfast ? e % TK : e / TM;
         const uint32_t gi = m0 + ii, gk = k0 + kk;
-        sA[ii][kk] = (gi < M && gk < K) ? A[gi * sai + gk * sak] : (__bf16)0.f;
+        sA[ii][kk] = (gi < M && gk < kb1) ? A[gi * sai + gk * sak] : (__bf16)0.f;
       }
     }
-    if (b_vec && n0 + TN <= N && k0 + TK <= K) {
-      // B(k, j..j+7) contiguous: 8 consecutive j of one k, transposed into sB
-      const int kb = tid / 8, jj = (tid % 8) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(B + (uint64_t)(k0 + kb) * sbk + n0 + jj);
-      const __bf16* e8 = reinterpret_cast<const __bf16*>(&v);
+    if (b_fast(k0)) {
+      const __bf16* e8 = reinterpret_cast<const __bf16*>(&rb);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) sB[jj + q][kb] = e8[q];
+      for (int q = 0; q < 8; ++q) sB[bj + q][bk] = e8[q];
     } else {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int e = tid + 256 * r;
         const int jj = b_jfast ? e % TN : e / TK, kb = b_jfast ? e / TN : e % TK;
         const uint32_t gj = n0 + jj, gkb = k0 + kb;
-        sB[jj][kb] = (gj < N && gkb < K) ? B[gkb * sbk + gj * sbj] : (__bf16)0.f;
+        sB[jj][kb] = (gj < N && gkb < kb1) ? B[gkb * sbk + gj * sbj] : (__bf16)0.f;
       }
     }
     __syncthreads();
+    if (k0 + TK < kb1) fetch(k0 + TK);  // in flight during this step's products
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       v8bf av;
@@ -115,13 +140,24 @@ __global__ __launch_bounds__(256) void gemm_bf16(const __bf16* __restrict__ A, i
         const uint32_t i = m0 + wm + 16 * a + 4 * q4 + r;
         const uint32_t j = n0 + wn + 16 * b + i16;
         if (i >= M || j >= N) continue;
-        float z = acc[a][b][r] + (bias ? bias[j] : 0.f);
-        if (Cf) {
-          const uint64_t o = (uint64_t)i * ldc + j;
-          Cf[o] = beta != 0.f ? z + beta * Cf[o] : z;  // FP32 output (pre-activation)
-        }
-        if (Cb) Cb[(uint64_t)i * ldc + j] = (__bf16)(act == 1 ? gelu(z) : z);
+        if (part)
+          part[((uint64_t)blockIdx.z * M + i) * N + j] = acc[a][b][r];
+        else
+          gemm_out(acc[a][b][r], i, j, bias, act, Cf, Cb, ldc, beta);
       }
+}
+
+// Sum of the K splits (in split order) and the epilogue.
+__global__ void gemm_reduce(const float* __restrict__ part, uint32_t splits, uint32_t M,
+                            uint32_t N, const float* __restrict__ bias, int act,
+                            float* __restrict__ Cf, __bf16* __restrict__ Cb, uint32_t ldc,
+                            float beta) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t mn = (uint64_t)M * N;
+  if (e >= mn) return;
+  float s = part[e];
+  for (uint32_t z = 1; z < splits; ++z) s += part[z * mn + e];
+  gemm_out(s, (uint32_t)(e / N), (uint32_t)(e % N), bias, act, Cf, Cb, ldc, beta);
 }
 
 // dZ = dY * act'(Z) (BF16 out), Z the saved FP32 pre-activation.
@@ -183,11 +219,51 @@ uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t)
 
 }  // namespace
 
+// K splits: enough workgroups to cover the CUs (>= 256) while every split
+// keeps at least 8 K steps; the partial sums live in a per-device workspace
+// that grows on demand (hipMalloc outside any capture; the GEMMs of the
+// network are not captured).
 hipError_t mlp_gemm(const MlpGemm& g, hipStream_t st) {
   if (g.M == 0 || g.N == 0) return hipSuccess;
-  const dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
+  const uint32_t tiles = ((g.N + TN - 1) / TN) * ((g.M + TM - 1) / TM);
+  const uint32_t ksteps = (g.K + TK - 1) / TK;
+  uint32_t splits = 1;
+  while (tiles * splits < 256 && ksteps / (splits + 1) >= 8) ++splits;
+  const uint32_t kchunk = ((ksteps + splits - 1) / splits) * TK;
+  splits = (g.K + kchunk - 1) / kchunk;
+  float* part = nullptr;
+  if (splits > 1) {
+    static std::mutex mu;
+    static void* ws[64] = {};
+    static size_t ws_bytes[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    const size_t need = (size_t)splits * g.M * g.N * sizeof(float);
+    std::lock_guard<std::mutex> lock(mu);
+    if (ws_bytes[dev] < need) {
+      if (ws[dev]) {
+        e = hipFree(ws[dev]);  // synchronizes: earlier GEMMs are done with it
+        if (e != hipSuccess) return e;
+        ws[dev] = nullptr;
+        ws_bytes[dev] = 0;
+      }
+      e = hipMalloc(&ws[dev], need);
+      if (e != hipSuccess) return e;
+      ws_bytes[dev] = need;
+    }
+    part = static_cast<float*>(ws[dev]);
+  }
+  const dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
   hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, st, g.A, g.sai, g.sak, g.B, g.sbk, g.sbj,
-                     g.M, g.N, g.K, g.bias, g.act, g.Cf, g.Cb, g.ldc, g.beta);
+                     g.M, g.N, g.K, splits > 1 ? kchunk : g.K, g.bias, g.act, g.Cf, g.Cb, g.ldc,
+                     g.beta, part);
+  if (splits > 1) {
+    const uint64_t mn = (uint64_t)g.M * g.N;
+    hipLaunchKernelGGL(gemm_reduce, dim3(blocks_for(mn, 256)), dim3(256), 0, st, part, splits,
+                       g.M, g.N, g.bias, g.act, g.Cf, g.Cb, g.ldc, g.beta);
+  }
   return hipGetLastError();
 }
 
