@@ -269,7 +269,12 @@ int mega_learned_finish(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const 
 /* Network kernels (BF16 operands, FP32 accumulation): C (+)= A B + bias,
  * activation (0 identity, 1 GELU tanh form), A(i,k) = A[i*sai + k*sak],
  * B(k,j) = B[k*sbk + j*sbj]; C row-major (ldc) in FP32 (Cf, pre-activation,
- * Cf = z + beta*Cf) and/or BF16 (Cb, after the activation). */
+ * Cf = z + beta*Cf) and/or BF16 (Cb, after the activation). Small M x N
+ * grids split K: the partial sums go through a library-owned workspace per
+ * device (allocated on first need, grown with hipMalloc), so GEMMs that
+ * overlap on one device (different streams) must be serialized by the
+ * caller, and the call must not be captured into a graph when it has to
+ * grow the workspace. */
 typedef struct {
   const void *A;
   int64_t sai, sak;
