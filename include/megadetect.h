@@ -261,10 +261,12 @@ int mega_learned_refine(uint32_t num_ue, uint32_t batch, const void *G, const vo
 
 /* W = X A per group (CSI layout), where flagged groups first replace X by
  * the plain sweep inverse of their Gram matrix (the fallback); X is updated
- * in place. info_dev (optional) gets the fallback's status. */
+ * in place. G (optional, row-major [batch][U][U]) is that Gram matrix as
+ * mega_gram_inv computed it for this slot; NULL recomputes it from csi.
+ * info_dev (optional) gets the fallback's status. */
 int mega_learned_finish(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const void *csi,
-                        void *X, const int32_t *flag, void *w_out, int32_t *info_dev,
-                        mega_stream_t stream);
+                        const void *G, void *X, const int32_t *flag, void *w_out,
+                        int32_t *info_dev, mega_stream_t stream);
 
 /* Network kernels (BF16 operands, FP32 accumulation): C (+)= A B + bias,
  * activation (0 identity, 1 GELU tanh form), A(i,k) = A[i*sai + k*sak],

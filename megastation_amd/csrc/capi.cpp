@@ -359,15 +359,17 @@ int mega_learned_refine(uint32_t num_ue, uint32_t batch, const void* G, const vo
                     "mega_learned_refine");
 }
 
-int mega_learned_finish(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const void* csi, void* X,
-                        const int32_t* flag, void* w_out, int32_t* info_dev, mega_stream_t stream) {
+int mega_learned_finish(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const void* csi,
+                        const void* G, void* X, const int32_t* flag, void* w_out,
+                        int32_t* info_dev, mega_stream_t stream) {
   if (batch == 0) return MEGA_OK;
   if (!csi || !X || !flag || !w_out) return fail(MEGA_ERR_INVALID_ARG, "mega_learned_finish: NULL buffer");
   if (num_ue == 0 || num_bs == 0) return fail(MEGA_ERR_INVALID_ARG, "mega_learned_finish: zero size");
   if (num_ue > mega::detail::kMaxUsers) return fail(MEGA_ERR_UNSUPPORTED, "mega_learned_finish: num_ue > 64");
   return hip_status(mega::detail::launch_learned_finish(
-                        num_ue, num_bs, batch, static_cast<const float2*>(csi), static_cast<float2*>(X),
-                        flag, static_cast<float2*>(w_out), info_dev, static_cast<hipStream_t>(stream)),
+                        num_ue, num_bs, batch, static_cast<const float2*>(csi),
+                        static_cast<const float2*>(G), static_cast<float2*>(X), flag,
+                        static_cast<float2*>(w_out), info_dev, static_cast<hipStream_t>(stream)),
                     "mega_learned_finish");
 }
 

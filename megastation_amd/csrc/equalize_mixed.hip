@@ -136,6 +136,122 @@ __global__ __launch_bounds__(256) void equalize_mixed(const EqParams p, const ui
   eq_unit_mixed<MT, T, R, NST, Q>(p, Ws, slot, g, T * t, mg);
 }
 
+// W-in-registers form for small cells (MT * NSW <= 8: W of the wave's users
+// and all antennas is at most 64 registers): one wave per (slot, group,
+// chunk of column tiles, group of 16*MT users) loads its W planes once and
+// streams y for every (tile, antenna step) of the chunk through a ring of
+// R register sets (R divides NSW, so every ring index is static; the
+// prefetch runs across tile boundaries). Same products, order and epilogue
+// as eq_unit_mixed.
+template <int MT, int NSW, int R, int Q>
+__device__ __forceinline__ void eq_unit_mixed_reg(const EqParams& p,
+                                                  const uint32_t* __restrict__ Ws,
+                                                  uint32_t slot, uint32_t g, uint32_t t0,
+                                                  uint32_t t1, uint32_t mg) {
+  static_assert(NSW % R == 0, "ring");
+  const int lane = threadIdx.x & 63;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const uint32_t U = p.U, B = p.B, N = p.N;
+  const uint32_t Bp = split_row(B);
+  const uint32_t k0 = g * p.sg;
+  const uint32_t nk = min(p.sg, N - k0);
+  const uint32_t* Wg = Ws + 2 * (slot * p.csi_slot + (uint64_t)g * U * Bp);
+  const uint64_t lo_off = (uint64_t)U * Bp;
+  v4u wh[MT][NSW], wl[MT][NSW];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const uint32_t* wrow = Wg + (uint64_t)min(16 * (mg * MT + m) + i16, U - 1) * Bp + 4 * q4;
+#pragma unroll
+    for (int st = 0; st < NSW; ++st) {
+      wh[m][st] = *reinterpret_cast<const v4u*>(wrow + 16 * st);
+      wl[m][st] = *reinterpret_cast<const v4u*>(wrow + lo_off + 16 * st);
+    }
+  }
+  auto ycol = [&](uint32_t t) {  // column pointer of tile t (clamped into the chunk)
+    const uint32_t n = min(16 * min(t, t1 - 1) + i16, nk * p.S - 1);
+    const uint32_t sym = n / nk;
+    return p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
+  };
+  float2 yr[R][4];
+  auto load = [&](const float2* yc, int st, float2 (&yd)[4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) yd[a] = yc[(uint64_t)min(16 * st + 4 * q4 + a, (int)B - 1) * N];
+  };
+  const float2* ycur = ycol(t0);
+  const float2* ynext = ycol(t0 + 1);
+#pragma unroll
+  for (int r = 0; r + 1 < R; ++r) load(ycur, r, yr[r]);
+  for (uint32_t t = t0; t < t1; ++t) {
+    v4f are[MT], aim[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) are[m] = aim[m] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < NSW; ++st) {
+      const int ps = st + R - 1;  // step to prefetch (may be in the next tile)
+      load(ps < NSW ? ycur : ynext, ps % NSW, yr[ps % R]);
+      __builtin_amdgcn_sched_barrier(0);
+      v4u b1h, b1l, b2h, b2l;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const bool ok = 16 * st + 4 * q4 + a < (int)B;
+        const SplitC sp = split_c(ok ? yr[st % R][a] : make_float2(0.f, 0.f));
+        b1h[a] = sp.hi ^ 0x80000000u;
+        b1l[a] = sp.lo ^ 0x80000000u;
+        b2h[a] = __builtin_rotateleft32(sp.hi, 16);
+        b2l[a] = __builtin_rotateleft32(sp.lo, 16);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        are[m] = mfma_x3(wh[m][st], wl[m][st], b1h, b1l, are[m]);
+        aim[m] = mfma_x3(wh[m][st], wl[m][st], b2h, b2l, aim[m]);
+      }
+    }
+    ycur = ynext;
+    ynext = ycol(t + 2);
+    if (16 * t < nk * p.S) {  // uniform: a tile of the group
+      const EqCol col = eq_col<Q>(p, slot, g, t);
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const uint32_t u0 = 16 * (mg * MT + m) + 4 * q4;
+        eq_store<Q>(p, col, u0, v4f{are[m][0], aim[m][0], are[m][1], aim[m][1]});
+        eq_store<Q>(p, col, u0 + 2, v4f{are[m][2], aim[m][2], are[m][3], aim[m][3]});
+      }
+    }
+  }
+}
+
+// units = slots * G * chunks * nmg, chunk c of a group = tiles
+// [c * tiles_per_wg, min((c+1) * tiles_per_wg, col_tiles)).
+template <int MT, int NSW, int R, int Q>
+__global__ __launch_bounds__(256) void equalize_mixed_reg(const EqParams p, const uint32_t* Ws,
+                                                          uint32_t units) {
+  const uint32_t unit = xcd_linear(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (unit >= units) return;
+  const uint32_t mg = unit % p.nmg;
+  const uint32_t c = (unit / p.nmg) % p.chunks;
+  const uint32_t rest = unit / p.nmg / p.chunks;
+  const uint32_t g = rest % p.G, slot = rest / p.G;
+  const uint32_t ntiles = (min(p.sg, p.N - g * p.sg) * p.S + 15) / 16;
+  const uint32_t t0 = c * p.tiles_per_wg, t1 = min(t0 + p.tiles_per_wg, ntiles);
+  if (t0 >= t1) return;  // past a partial group
+  eq_unit_mixed_reg<MT, NSW, R, Q>(p, Ws, slot, g, t0, t1, mg);
+}
+
+template <int MT, int NSW>
+hipError_t launch_reg(const EqParams& p, const uint32_t* wp, uint32_t units, uint32_t Q,
+                      hipStream_t st) {
+  const dim3 grid((units + 3) / 4), block(256);
+  switch (Q) {
+    case 2: hipLaunchKernelGGL((equalize_mixed_reg<MT, NSW, 4, 2>), grid, block, 0, st, p, wp, units); break;
+    case 4: hipLaunchKernelGGL((equalize_mixed_reg<MT, NSW, 4, 4>), grid, block, 0, st, p, wp, units); break;
+    case 6: hipLaunchKernelGGL((equalize_mixed_reg<MT, NSW, 4, 6>), grid, block, 0, st, p, wp, units); break;
+    case 8: hipLaunchKernelGGL((equalize_mixed_reg<MT, NSW, 4, 8>), grid, block, 0, st, p, wp, units); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int MT, int T, int R, int NST>
 hipError_t launch_q(const EqParams& p, const uint32_t* wp, uint32_t units, uint32_t Q,
                     hipStream_t st) {
@@ -185,6 +301,31 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStrea
   uint32_t MT = a.U <= 16 ? 1 : 2;
   const int smt = env_int("MEGA_EQ_XMT");
   if (smt == 1 || smt == 2) MT = (uint32_t)smt;
+  // W in registers when the wave's W fits in 64 registers (split rows of at
+  // most 128 antennas; MEGA_EQ_XREG=1 forces streaming): W is read once
+  // per wave for a chunk of the group's tiles, chunks sized so the grid has
+  // at least 4096 waves.
+  const uint32_t nsw = split_row(a.B) / 16;
+  const int xreg = env_int("MEGA_EQ_XREG");
+  if (xreg != 1 && (nsw == 4 || nsw == 8) && MT * nsw <= 8) {
+    p.nmg = (a.U + 16 * MT - 1) / (16 * MT);
+    p.nwk = 1;
+    const uint64_t base = (uint64_t)a.slots * a.G * p.nmg;
+    uint32_t chunks = 1;
+    while (base * chunks < 4096 && chunks < p.col_tiles) ++chunks;
+    p.tiles_per_wg = (p.col_tiles + chunks - 1) / chunks;
+    p.chunks = (p.col_tiles + p.tiles_per_wg - 1) / p.tiles_per_wg;
+    const uint64_t units = base * p.chunks;
+    if (units > 0xffffffffull - 3) {
+      *why = "grid too large";
+      return hipErrorInvalidValue;
+    }
+    const uint32_t* w = static_cast<const uint32_t*>(wp);
+    const uint32_t n = (uint32_t)units;
+    if (MT == 1)
+      return nsw == 4 ? launch_reg<1, 4>(p, w, n, a.Q, st) : launch_reg<1, 8>(p, w, n, a.Q, st);
+    return launch_reg<2, 4>(p, w, n, a.Q, st);
+  }
   // Column tiles per wave: 2 (one W stream for 32 columns) when a group has
   // more than one tile, else 1; MEGA_EQ_XT=1|2 overrides.
   uint32_t T = p.col_tiles > 1 ? 2 : 1;

@@ -66,8 +66,8 @@ hipError_t launch_learned_refine(uint32_t U, uint32_t batch, const float2* G, co
                                  const float* scale, float2* X_out, float* res, int32_t* flag,
                                  float bound, uint32_t iters, hipStream_t stream);
 hipError_t launch_learned_finish(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
-                                 float2* X, const int32_t* flag, float2* w, int32_t* info,
-                                 hipStream_t stream);
+                                 const float2* Gm, float2* X, const int32_t* flag, float2* w,
+                                 int32_t* info, hipStream_t st);
 
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,

@@ -66,11 +66,14 @@ __global__ __launch_bounds__(256) void features_kernel(const float2* __restrict_
   __shared__ float s_sh;
   const uint32_t g = blockIdx.x;
   const uint64_t o = (uint64_t)g * U * U;
-  if (threadIdx.x == 0) {
-    float tr = 0.f;
-    for (uint32_t i = 0; i < U; ++i) tr += Gp[o + i * U + i].x;
-    s_sh = tr > 0.f ? tr / (float)U : 1.f;
-    scale[g] = s_sh;
+  if (threadIdx.x < 64) {  // trace of G_prev: one diagonal entry per lane (U <= 64)
+    float tr = threadIdx.x < U ? Gp[o + threadIdx.x * (U + 1)].x : 0.f;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tr += __shfl_xor(tr, off, 64);
+    if (threadIdx.x == 0) {
+      s_sh = tr > 0.f ? tr / (float)U : 1.f;
+      scale[g] = s_sh;
+    }
   }
   __syncthreads();
   const float s = s_sh, inv = 1.f / s;
@@ -106,97 +109,124 @@ __global__ __launch_bounds__(256) void target_kernel(const float2* __restrict__ 
   }
 }
 
-// C = A B for U x U complex matrices in LDS (row-major, ld U), all threads.
-__device__ __forceinline__ void cmatmul(const float2* A, const float2* B, float2* C, uint32_t U) {
-  for (uint32_t e = threadIdx.x; e < U * U; e += blockDim.x) {
-    const uint32_t i = e / U, c = e % U;
-    float re = 0.f, im = 0.f;
-    for (uint32_t j = 0; j < U; ++j) {
-      const float2 a = A[i * U + j], b = B[j * U + c];
-      re += a.x * b.x - a.y * b.y;
-      im += a.x * b.y + a.y * b.x;
+// Products of UP x UP complex matrices in LDS (rows of UP+1, zero past U).
+// Thread layout as the sweep's: column c = tid % UP of rows r0 + RS*j, so a
+// thread reads B[k][c] once per k for its EPT rows.
+template <int UP>
+struct RefineLayout {
+  static constexpr int NTH = 256, LD = UP + 1, RS = NTH / UP, EPT = UP * UP / NTH;
+  static_assert(NTH % UP == 0 && (UP * UP) % NTH == 0, "layout");
+};
+
+template <int UP>
+__device__ __forceinline__ void cmatmul(const float2* A, const float2* B, float2 (&acc)[RefineLayout<UP>::EPT]) {
+  using L = RefineLayout<UP>;
+  const int c = threadIdx.x % UP, r0 = threadIdx.x / UP;
+#pragma unroll
+  for (int j = 0; j < L::EPT; ++j) acc[j] = make_float2(0.f, 0.f);
+#pragma unroll 4
+  for (int k = 0; k < UP; ++k) {
+    const float2 b = B[k * L::LD + c];
+#pragma unroll
+    for (int j = 0; j < L::EPT; ++j) {
+      const float2 a = A[(r0 + L::RS * j) * L::LD + k];
+      acc[j].x += a.x * b.x - a.y * b.y;
+      acc[j].y += a.x * b.y + a.y * b.x;
     }
-    C[e] = make_float2(re, im);
   }
 }
 
-// Warm start, Newton-Schulz refinement and the residual judge; dynamic LDS
-// 4 U x U complex matrices.
+// Warm start, Newton-Schulz refinement and the residual judge (users padded
+// to UP with zeros; the identity only on the first U diagonal entries).
+template <int UP>
 __global__ __launch_bounds__(256) void refine_kernel(
     const float2* __restrict__ G, const float2* __restrict__ Gp, const float2* __restrict__ Xp,
     const float* __restrict__ corr, uint32_t corr_ld, const float* __restrict__ scale,
     float2* __restrict__ X_out, float* __restrict__ res, int32_t* __restrict__ flag,
     float bound, uint32_t iters, uint32_t U) {
-  extern __shared__ float2 sm[];
+  using L = RefineLayout<UP>;
+  constexpr int LD = L::LD, EPT = L::EPT, RS = L::RS;
+  extern __shared__ float2 sm[];  // G, X, T: UP x LD each (refine_lds)
   float2* sG = sm;
-  float2* sX = sm + U * U;
-  float2* t1 = sm + 2 * U * U;
-  float2* t2 = sm + 3 * U * U;
+  float2* sX = sm + UP * LD;
+  float2* sT = sm + 2 * UP * LD;
   __shared__ float red[4];
   const uint32_t g = blockIdx.x;
   const uint64_t o = (uint64_t)g * U * U;
-  const uint32_t n = U * U;
-  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const float2 a = G[o + e], b = Gp[o + e];
-    sG[e] = a;
-    t1[e] = make_float2(a.x - b.x, a.y - b.y);  // D
-    sX[e] = Xp[o + e];
+  const int c = threadIdx.x % UP, r0 = threadIdx.x / UP;
+  const bool cin = c < (int)U;
+  // own entries: G, D = G - G_prev, X_prev (zero past U)
+  float2 xv[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const int i = r0 + RS * j;
+    const bool in = cin && i < (int)U;
+    const float2 a = in ? G[o + i * U + c] : make_float2(0.f, 0.f);
+    const float2 b = in ? Gp[o + i * U + c] : make_float2(0.f, 0.f);
+    xv[j] = in ? Xp[o + i * U + c] : make_float2(0.f, 0.f);
+    sG[i * LD + c] = a;
+    sT[i * LD + c] = make_float2(a.x - b.x, a.y - b.y);  // D
+    sX[i * LD + c] = xv[j];
   }
   __syncthreads();
-  cmatmul(sX, t1, t2, U);  // Xp D
+  float2 acc[EPT];
+  cmatmul<UP>(sX, sT, acc);  // Xp D
   __syncthreads();
-  cmatmul(t2, sX, t1, U);  // Xp D Xp
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) sT[(r0 + RS * j) * LD + c] = acc[j];
   __syncthreads();
+  cmatmul<UP>(sT, sX, acc);  // Xp D Xp
   // X0 = Xp - Xp D Xp + C / s (C Hermitian from its upper triangle)
   const float inv = corr ? 1.f / scale[g] : 0.f;
   const float* cg = corr ? corr + (uint64_t)g * corr_ld : nullptr;
-  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const uint32_t i = e / U, c = e % U;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const int i = r0 + RS * j;
     float cr = 0.f, ci = 0.f;
-    if (cg) {
+    if (cg && cin && i < (int)U) {
       const uint32_t a = min(i, c), b = max(i, c);
       const uint32_t t = a * U - a * (a - 1) / 2 + (b - a);  // upper-triangle index
       cr = cg[2 * t] * inv;
-      ci = cg[2 * t + 1] * inv;
-      if (i > c) ci = -ci;
-      if (i == c) ci = 0.f;
+      ci = i > c ? -cg[2 * t + 1] * inv : i == c ? 0.f : cg[2 * t + 1] * inv;
     }
-    const float2 x = sX[e], d = t1[e];
-    t2[e] = make_float2(x.x - d.x + cr, x.y - d.y + ci);
+    xv[j] = make_float2(xv[j].x - acc[j].x + cr, xv[j].y - acc[j].y + ci);
+    sX[i * LD + c] = xv[j];
   }
-  __syncthreads();
-  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) sX[e] = t2[e];
   __syncthreads();
   for (uint32_t it = 0; it < iters; ++it) {
-    cmatmul(sG, sX, t1, U);  // G X
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-      const float2 v = t1[e];
-      t1[e] = make_float2((e / U == e % U ? 2.f : 0.f) - v.x, -v.y);  // 2I - G X
+    cmatmul<UP>(sG, sX, acc);  // G X
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int i = r0 + RS * j;
+      const float d = (i == c && cin) ? 2.f : 0.f;
+      sT[i * LD + c] = make_float2(d - acc[j].x, -acc[j].y);  // 2I - G X
     }
     __syncthreads();
-    cmatmul(sX, t1, t2, U);  // X (2I - G X)
+    cmatmul<UP>(sX, sT, acc);  // X (2I - G X)
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) sX[e] = t2[e];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      xv[j] = acc[j];
+      sX[(r0 + RS * j) * LD + c] = acc[j];
+    }
     __syncthreads();
   }
-  cmatmul(sX, sG, t1, U);  // X G
-  __syncthreads();
+  cmatmul<UP>(sX, sG, acc);  // X G
   float ss = 0.f;
-  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const float2 v = t1[e];
-    const float re = v.x - (e / U == e % U ? 1.f : 0.f);
-    ss += re * re + v.y * v.y;
-    X_out[o + e] = sX[e];
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const int i = r0 + RS * j;
+    const float re = acc[j].x - ((i == c && cin) ? 1.f : 0.f);
+    ss += re * re + acc[j].y * acc[j].y;
+    if (cin && i < (int)U) X_out[o + i * U + c] = xv[j];
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float tot = 0.f;
-    for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) tot += red[w];
-    const float r = sqrtf(tot);
+    const float r = sqrtf((red[0] + red[1]) + (red[2] + red[3]));
     res[g] = r;
     flag[g] = (r <= bound) ? 0 : 1;  // NaN fails too
   }
@@ -207,8 +237,8 @@ __global__ __launch_bounds__(256) void refine_kernel(
 // major [U][U]) is updated in place for the next slot.
 template <int UT>
 __global__ __launch_bounds__(lz_threads<UT>()) void finish_kernel(
-    const float2* __restrict__ in, float2* X, const int32_t* __restrict__ flag, float2* w_out,
-    int32_t* info, uint32_t U, uint32_t B) {
+    const float2* __restrict__ in, const float2* __restrict__ Gm, float2* X,
+    const int32_t* __restrict__ flag, float2* w_out, int32_t* info, uint32_t U, uint32_t B) {
   constexpr int NTH = lz_threads<UT>();
   constexpr int LDG = ZfShared<UT>::LDG;
   constexpr int UP = ZfShared<UT>::UP;
@@ -221,7 +251,15 @@ __global__ __launch_bounds__(lz_threads<UT>()) void finish_kernel(
   const uint64_t o = (uint64_t)g * U * U;
   int fail = 0;
   if (flag[g]) {
-    zf_gram<UT, NTH, false>(A, U, B, sh, nullptr, part);
+    if (Gm) {  // the Gram matrix computed earlier in the slot (row-major [U][U])
+      for (uint32_t e = threadIdx.x; e < (uint32_t)(UP * UP); e += NTH) {
+        const uint32_t i = e / UP, c = e % UP;
+        sh.G[i * LDG + c] = (i < U && c < U) ? Gm[o + i * U + c] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+    } else {
+      zf_gram<UT, NTH, false>(A, U, B, sh, nullptr, part);
+    }
     fail = zf_sweep<UT, NTH>(sh, U);
     for (uint32_t e = threadIdx.x; e < U * U; e += NTH) {
       const float2 v = sh.G[(e / U) * LDG + e % U];
@@ -266,8 +304,9 @@ hipError_t launch_gram_inv_ut(uint32_t U, uint32_t B, uint32_t batch, const floa
 }
 
 template <int UT>
-hipError_t launch_finish_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in, float2* X,
-                            const int32_t* flag, float2* w, int32_t* info, hipStream_t st) {
+hipError_t launch_finish_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                            const float2* Gm, float2* X, const int32_t* flag, float2* w,
+                            int32_t* info, hipStream_t st) {
   constexpr int NTH = lz_threads<UT>();
   const size_t lds = zf_dyn_lds<UT, NTH>(U, B, false);
   if (lds > 65536) {
@@ -275,8 +314,8 @@ hipError_t launch_finish_ut(uint32_t U, uint32_t B, uint32_t batch, const float2
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((finish_kernel<UT>), dim3(batch), dim3(NTH), lds, st, in, X, flag, w, info,
-                     U, B);
+  hipLaunchKernelGGL((finish_kernel<UT>), dim3(batch), dim3(NTH), lds, st, in, Gm, X, flag, w,
+                     info, U, B);
   return hipGetLastError();
 }
 
@@ -304,29 +343,44 @@ hipError_t launch_learned_target(uint32_t U, uint32_t batch, const float2* Xt, c
   return hipGetLastError();
 }
 
+template <int UP>
+hipError_t launch_refine_up(uint32_t U, uint32_t batch, const float2* G, const float2* Gp,
+                            const float2* Xp, const float* corr, uint32_t corr_ld,
+                            const float* scale, float2* X_out, float* res, int32_t* flag,
+                            float bound, uint32_t iters, hipStream_t st) {
+  const size_t lds = (size_t)3 * UP * (UP + 1) * sizeof(float2);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&refine_kernel<UP>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(refine_kernel<UP>, dim3(batch), dim3(256), lds, st, G, Gp, Xp, corr,
+                     corr_ld, scale, X_out, res, flag, bound, iters, U);
+  return hipGetLastError();
+}
+
 hipError_t launch_learned_refine(uint32_t U, uint32_t batch, const float2* G, const float2* Gp,
                                  const float2* Xp, const float* corr, uint32_t corr_ld,
                                  const float* scale, float2* X_out, float* res, int32_t* flag,
                                  float bound, uint32_t iters, hipStream_t st) {
   if (batch == 0) return hipSuccess;
-  const size_t lds = (size_t)4 * U * U * sizeof(float2);
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&refine_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(refine_kernel, dim3(batch), dim3(256), lds, st, G, Gp, Xp, corr, corr_ld,
-                     scale, X_out, res, flag, bound, iters, U);
-  return hipGetLastError();
+  if (U <= 16)
+    return launch_refine_up<16>(U, batch, G, Gp, Xp, corr, corr_ld, scale, X_out, res, flag,
+                                bound, iters, st);
+  if (U <= 32)
+    return launch_refine_up<32>(U, batch, G, Gp, Xp, corr, corr_ld, scale, X_out, res, flag,
+                                bound, iters, st);
+  return launch_refine_up<64>(U, batch, G, Gp, Xp, corr, corr_ld, scale, X_out, res, flag,
+                              bound, iters, st);
 }
 
 hipError_t launch_learned_finish(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
-                                 float2* X, const int32_t* flag, float2* w, int32_t* info,
-                                 hipStream_t st) {
+                                 const float2* Gm, float2* X, const int32_t* flag, float2* w,
+                                 int32_t* info, hipStream_t st) {
   if (batch == 0) return hipSuccess;
-  if (U <= 16) return launch_finish_ut<1>(U, B, batch, in, X, flag, w, info, st);
-  if (U <= 32) return launch_finish_ut<2>(U, B, batch, in, X, flag, w, info, st);
-  return launch_finish_ut<4>(U, B, batch, in, X, flag, w, info, st);
+  if (U <= 16) return launch_finish_ut<1>(U, B, batch, in, Gm, X, flag, w, info, st);
+  if (U <= 32) return launch_finish_ut<2>(U, B, batch, in, Gm, X, flag, w, info, st);
+  return launch_finish_ut<4>(U, B, batch, in, Gm, X, flag, w, info, st);
 }
 
 }  // namespace detail

@@ -50,7 +50,7 @@ def _L():
             "mega_learned_features": [u32, u32, P, P, P, P, P, P],
             "mega_learned_target": [u32, u32, P, P, P, P, P],
             "mega_learned_refine": [u32, u32, P, P, P, P, u32, P, P, P, P, f32, u32, P],
-            "mega_learned_finish": [u32, u32, u32, P, P, P, P, P, P],
+            "mega_learned_finish": [u32, u32, u32, P, P, P, P, P, P, P],
             "mega_mlp_gemm": [ctypes.POINTER(GemmDesc), P],
             "mega_mlp_act_backward": [P, P, P, P, u64, i32, P],
             "mega_mlp_col_sum": [P, P, u32, u32, P],
@@ -276,10 +276,12 @@ def target(X_true, X_base, scale, U: int, stream=None):
     return t
 
 
-def finish(A, X, flag, U: int, B: int, w_out, info=None, stream=None):
+def finish(A, X, flag, U: int, B: int, w_out, info=None, stream=None, G=None):
+    """W = X A; flagged groups take the plain sweep of their Gram matrix
+    (G, this slot's from gram_inv, or recomputed from A when None)."""
     n = X.shape[0]
-    _check(_L().mega_learned_finish(B, U, n, _ptr(A), _ptr(X), _ptr(flag), _ptr(w_out),
-                                    _ptr(info), _stream(stream)))
+    _check(_L().mega_learned_finish(B, U, n, _ptr(A), _ptr(G), _ptr(X), _ptr(flag),
+                                    _ptr(w_out), _ptr(info), _stream(stream)))
 
 
 def training_pairs(chan: GpuChannel, t0: int, n_pairs: int, U: int, B: int):
@@ -369,7 +371,7 @@ class LearnedZF:
             corr = self.net.forward(f, stream=stream)
         refine(Gt, Gp, Xp, U, corr, scale, self.bound, self.iters, stream=stream,
                out=(X, self.res, self.flag))
-        finish(A, X, self.flag, U, B, self.W, self.info, stream=stream)
+        finish(A, X, self.flag, U, B, self.W, self.info, stream=stream, G=Gt)
         self.k = cur
         self.last = {"flag": self.flag, "residual": self.res}
         return self.W, self.info

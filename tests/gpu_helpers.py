@@ -89,11 +89,11 @@ XHAT_RTOL = 1e-4
 HARD_BIT_MARGIN = 0.05
 
 
-def check_symbol(llr_gpu, xhat_gpu, llr64, xhat64, N, sg, Q, lay):
+def check_symbol(llr_gpu, xhat_gpu, llr64, xhat64, N, sg, Q, lay, rtol=XHAT_RTOL):
     """One symbol: llr [U][ue_stride] uint16, xhat [U][N]."""
     scale = np.abs(xhat64).max()
     err = np.abs(xhat_gpu.astype(np.complex128) - xhat64).max()
-    assert err <= XHAT_RTOL * scale, (err, scale)
+    assert err <= rtol * scale, (err, scale)
     own = oracle_llr_from_xhat(xhat_gpu, N, sg, Q, lay)
     mism = np.flatnonzero(own.reshape(-1) != llr_gpu.reshape(-1))
     assert mism.size == 0, f"{mism.size} LLRs differ from demod of own x_hat, first at {mism[:5]}"

@@ -161,6 +161,11 @@ def test_finish_fallback_and_apply():
     wz = host_c64(W_zf).reshape(G, B * U)
     f = flag.cpu().numpy().astype(bool)
     assert np.array_equal(w[f], wz[f])                       # fallback = plain ZF
+    # the slot's stored Gram matrix instead of a recomputed one: same W
+    W2 = t.empty_like(W)
+    learned.finish(d_a, X.clone(), flag, U, B, W2, info, G=Gm)
+    t.cuda.synchronize()
+    assert np.array_equal(host_c64(W2), host_c64(W))
     # unflagged: W = X A with the given X
     A = a.reshape(G, B, U).astype(np.complex128)
     xn = host_c64(X).astype(np.complex128)
