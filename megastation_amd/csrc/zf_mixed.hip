@@ -24,6 +24,8 @@
 namespace mega {
 namespace detail {
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr float kCondMax = 1e3f;  // kappa_1(G) above this takes the FP64 route
 
 namespace {
@@ -355,51 +357,93 @@ __global__ __launch_bounds__(kMixThreads, UT >= 4 ? 1 : 2) void zf_mixed(const f
 
   if (!fp64) {
     // ---- 3. refinement: R = I - G X0 (FP64), X = X0 + X0 R -----------------
-    constexpr int EPT = UP * UP / NTH;
-    constexpr int RS = NTH / UP;
-    const int c = tid % UP, r0 = tid / UP;
-    float2 rr[EPT];
+    // Both products on the matrix cores, one 16 x 16 output tile per wave and
+    // step (v_mfma_f64_16x16x4_f64 for R: FP32 operands are exact in FP64,
+    // products and sums in FP64; v_mfma_f32_16x16x4_f32 for X). Operand lane
+    // l: A[row l%16][k l/16], B[k l/16][col l%16]; result lane l, register r:
+    // column l%16 of row 4*(l/16)+r (FP32) or row l/16+4*r (FP64, its result
+    // rows are interleaved). sG holds -X0; entries past U are zero in
+    // G0 and off the diagonal in sG, so k runs over UP.
+    {
+      constexpr int TT = UT * UT;  // output tiles
+      constexpr int KS = UP / 4;   // k steps
+      // R = I + G0 (-X0), rounded to FP32, masked to the U x U block
+      float2 rr[(TT + NW - 1) / NW][4];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int i = r0 + RS * j;
-      double sr = (i == c) ? 1.0 : 0.0, si = 0.0;
+      for (int q = 0; q < (TT + NW - 1) / NW; ++q) {
+        const int tile = wave + NW * q;
+        if (tile >= TT) break;  // uniform per wave
+        const int ti = tile / UT, tj = tile % UT;
+        v4d cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
 #pragma unroll
-      for (int k = 0; k < UP; ++k) {  // G0 is zero past U
-        const float2 gk = sh.G0[i * LDG + k];
-        const float2 xk = sG[k * LDG + c];  // -X0
-        sr += (double)gk.x * xk.x - (double)gk.y * xk.y;
-        si += (double)gk.x * xk.y + (double)gk.y * xk.x;
+        for (int ks = 0; ks < KS; ++ks) {
+          const float2 a = sh.G0[(16 * ti + i16) * LDG + 4 * ks + q4];
+          const float2 b = sG[(4 * ks + q4) * LDG + 16 * tj + i16];
+          const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+          cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre, 0, 0, 0);
+          cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cre, 0, 0, 0);
+          cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, cim, 0, 0, 0);
+          cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, cim, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * ti + q4 + 4 * r, c = 16 * tj + i16;  // FP64 result rows
+          const bool in = i < (int)U && c < (int)U;
+          rr[q][r] = in ? make_float2((float)((i == c ? 1.0 : 0.0) + cre[r]), (float)cim[r])
+                        : make_float2(0.f, 0.f);
+        }
       }
-      rr[j] = (i < (int)U && c < (int)U) ? make_float2((float)sr, (float)si) : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
+      __syncthreads();  // every read of G0 is done
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) sh.G0[(r0 + RS * j) * LDG + c] = rr[j];
-    __syncthreads();
-    float2 xn[EPT];
+      for (int q = 0; q < (TT + NW - 1) / NW; ++q) {
+        const int tile = wave + NW * q;
+        if (tile >= TT) break;
+        const int ti = tile / UT, tj = tile % UT;
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int i = r0 + RS * j;
-      const float2 x0 = sG[i * LDG + c];
-      float xr = -x0.x, xi = -x0.y;
-#pragma unroll
-      for (int k = 0; k < UP; ++k) {  // -X0 is zero past U off its diagonal, R past U
-        const float2 xk = sG[i * LDG + k];
-        const float2 rk = sh.G0[k * LDG + c];
-        xr -= xk.x * rk.x - xk.y * rk.y;
-        xi -= xk.x * rk.y + xk.y * rk.x;
+        for (int r = 0; r < 4; ++r)
+          sh.G0[(16 * ti + q4 + 4 * r) * LDG + 16 * tj + i16] = rr[q][r];
       }
-      xn[j] = make_float2(xr, xi);
-    }
-    __syncthreads();
-    // split X into BF16 words for the apply (G0 reused as uint2 storage)
-    uint2* sX = reinterpret_cast<uint2*>(sh.G0);
+      __syncthreads();
+      // X = X0 + X0 R = -((-X0) + (-X0) R), split into BF16 words for the apply
+      float2 xn[(TT + NW - 1) / NW][4];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const SplitC s = split_c(xn[j]);
-      sX[(r0 + RS * j) * LDG + c] = make_uint2(s.hi, s.lo);
+      for (int q = 0; q < (TT + NW - 1) / NW; ++q) {
+        const int tile = wave + NW * q;
+        if (tile >= TT) break;
+        const int ti = tile / UT, tj = tile % UT;
+        v4f cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const float2 a = sG[(16 * ti + i16) * LDG + 4 * ks + q4];
+          const float2 b = sh.G0[(4 * ks + q4) * LDG + 16 * tj + i16];
+          cre = mfma4(a.x, b.x, cre);
+          cre = mfma4(-a.y, b.y, cre);
+          cim = mfma4(a.x, b.y, cim);
+          cim = mfma4(a.y, b.x, cim);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float2 x0 = sG[(16 * ti + 4 * q4 + r) * LDG + 16 * tj + i16];
+          xn[q][r] = make_float2(-(x0.x + cre[r]), -(x0.y + cim[r]));
+        }
+      }
+      __syncthreads();  // every read of G0 (R) is done
+      // split X into BF16 words for the apply (G0 reused as uint2 storage)
+      uint2* sXw = reinterpret_cast<uint2*>(sh.G0);
+#pragma unroll
+      for (int q = 0; q < (TT + NW - 1) / NW; ++q) {
+        const int tile = wave + NW * q;
+        if (tile >= TT) break;
+        const int ti = tile / UT, tj = tile % UT;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const SplitC sp = split_c(xn[q][r]);
+          sXw[(16 * ti + 4 * q4 + r) * LDG + 16 * tj + i16] = make_uint2(sp.hi, sp.lo);
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
+    const uint2* sX = reinterpret_cast<const uint2*>(sh.G0);
 
     // ---- 4. W = X A on split BF16 products ----------------------------------
     // A tiles of 16 antennas x all users; a wave's tiles are loaded in
