@@ -109,30 +109,35 @@ __global__ __launch_bounds__(256) void target_kernel(const float2* __restrict__ 
   }
 }
 
-// Products of UP x UP complex matrices in LDS (rows of UP+1, zero past U).
-// Thread layout as the sweep's: column c = tid % UP of rows r0 + RS*j, so a
-// thread reads B[k][c] once per k for its EPT rows.
+// Products of UP x UP complex matrices in LDS (rows of UP+1, zero past U) on
+// the FP32 matrix cores: one 16 x 16 output tile per wave and step
+// (v_mfma_f32_16x16x4_f32, four real products per complex step). Operand
+// lane l: A[row l%16][k l/16], B[k l/16][col l%16]; result lane l, register
+// r: row 4*(l/16)+r, column l%16. epi(row, col, re, im) consumes each entry.
 template <int UP>
 struct RefineLayout {
-  static constexpr int NTH = 256, LD = UP + 1, RS = NTH / UP, EPT = UP * UP / NTH;
-  static_assert(NTH % UP == 0 && (UP * UP) % NTH == 0, "layout");
+  static constexpr int NTH = 256, LD = UP + 1, NW = NTH / 64, TT = (UP / 16) * (UP / 16);
 };
 
-template <int UP>
-__device__ __forceinline__ void cmatmul(const float2* A, const float2* B, float2 (&acc)[RefineLayout<UP>::EPT]) {
+template <int UP, typename Epi>
+__device__ __forceinline__ void cmm(const float2* A, const float2* B, Epi&& epi) {
   using L = RefineLayout<UP>;
-  const int c = threadIdx.x % UP, r0 = threadIdx.x / UP;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  for (int tile = wave; tile < L::TT; tile += L::NW) {  // uniform per wave
+    const int ti = tile / (UP / 16), tj = tile % (UP / 16);
+    v4f cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int j = 0; j < L::EPT; ++j) acc[j] = make_float2(0.f, 0.f);
-#pragma unroll 4
-  for (int k = 0; k < UP; ++k) {
-    const float2 b = B[k * L::LD + c];
-#pragma unroll
-    for (int j = 0; j < L::EPT; ++j) {
-      const float2 a = A[(r0 + L::RS * j) * L::LD + k];
-      acc[j].x += a.x * b.x - a.y * b.y;
-      acc[j].y += a.x * b.y + a.y * b.x;
+    for (int ks = 0; ks < UP / 4; ++ks) {
+      const float2 a = A[(16 * ti + i16) * L::LD + 4 * ks + q4];
+      const float2 b = B[(4 * ks + q4) * L::LD + 16 * tj + i16];
+      cre = mfma4(a.x, b.x, cre);
+      cre = mfma4(-a.y, b.y, cre);
+      cim = mfma4(a.x, b.y, cim);
+      cim = mfma4(a.y, b.x, cim);
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) epi(16 * ti + 4 * q4 + r, 16 * tj + i16, cre[r], cim[r]);
   }
 }
 
@@ -145,81 +150,67 @@ __global__ __launch_bounds__(256) void refine_kernel(
     float2* __restrict__ X_out, float* __restrict__ res, int32_t* __restrict__ flag,
     float bound, uint32_t iters, uint32_t U) {
   using L = RefineLayout<UP>;
-  constexpr int LD = L::LD, EPT = L::EPT, RS = L::RS;
-  extern __shared__ float2 sm[];  // G, X, T: UP x LD each (refine_lds)
+  constexpr int LD = L::LD;
+  extern __shared__ float2 sm[];  // G, X, T, V: UP x LD each
   float2* sG = sm;
   float2* sX = sm + UP * LD;
   float2* sT = sm + 2 * UP * LD;
+  float2* sV = sm + 3 * UP * LD;
   __shared__ float red[4];
   const uint32_t g = blockIdx.x;
   const uint64_t o = (uint64_t)g * U * U;
-  const int c = threadIdx.x % UP, r0 = threadIdx.x / UP;
-  const bool cin = c < (int)U;
-  // own entries: G, D = G - G_prev, X_prev (zero past U)
-  float2 xv[EPT];
-#pragma unroll
-  for (int j = 0; j < EPT; ++j) {
-    const int i = r0 + RS * j;
-    const bool in = cin && i < (int)U;
+  for (int e = threadIdx.x; e < UP * UP; e += L::NTH) {
+    const int i = e / UP, c = e % UP;
+    const bool in = i < (int)U && c < (int)U;
     const float2 a = in ? G[o + i * U + c] : make_float2(0.f, 0.f);
     const float2 b = in ? Gp[o + i * U + c] : make_float2(0.f, 0.f);
-    xv[j] = in ? Xp[o + i * U + c] : make_float2(0.f, 0.f);
     sG[i * LD + c] = a;
     sT[i * LD + c] = make_float2(a.x - b.x, a.y - b.y);  // D
-    sX[i * LD + c] = xv[j];
+    sX[i * LD + c] = in ? Xp[o + i * U + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  float2 acc[EPT];
-  cmatmul<UP>(sX, sT, acc);  // Xp D
+  auto store = [](float2* M) {
+    return [M](int i, int c, float re, float im) { M[i * LD + c] = make_float2(re, im); };
+  };
+  cmm<UP>(sX, sT, store(sV));  // Xp D
   __syncthreads();
-#pragma unroll
-  for (int j = 0; j < EPT; ++j) sT[(r0 + RS * j) * LD + c] = acc[j];
+  cmm<UP>(sV, sX, store(sT));  // Xp D Xp
   __syncthreads();
-  cmatmul<UP>(sT, sX, acc);  // Xp D Xp
   // X0 = Xp - Xp D Xp + C / s (C Hermitian from its upper triangle)
   const float inv = corr ? 1.f / scale[g] : 0.f;
   const float* cg = corr ? corr + (uint64_t)g * corr_ld : nullptr;
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < EPT; ++j) {
-    const int i = r0 + RS * j;
+  for (int e = threadIdx.x; e < UP * UP; e += L::NTH) {
+    const int i = e / UP, c = e % UP;
     float cr = 0.f, ci = 0.f;
-    if (cg && cin && i < (int)U) {
+    if (cg && i < (int)U && c < (int)U) {
       const uint32_t a = min(i, c), b = max(i, c);
       const uint32_t t = a * U - a * (a - 1) / 2 + (b - a);  // upper-triangle index
       cr = cg[2 * t] * inv;
       ci = i > c ? -cg[2 * t + 1] * inv : i == c ? 0.f : cg[2 * t + 1] * inv;
     }
-    xv[j] = make_float2(xv[j].x - acc[j].x + cr, xv[j].y - acc[j].y + ci);
-    sX[i * LD + c] = xv[j];
+    const float2 x = sX[i * LD + c], d = sT[i * LD + c];
+    sX[i * LD + c] = make_float2(x.x - d.x + cr, x.y - d.y + ci);
   }
   __syncthreads();
   for (uint32_t it = 0; it < iters; ++it) {
-    cmatmul<UP>(sG, sX, acc);  // G X
-#pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int i = r0 + RS * j;
-      const float d = (i == c && cin) ? 2.f : 0.f;
-      sT[i * LD + c] = make_float2(d - acc[j].x, -acc[j].y);  // 2I - G X
-    }
+    cmm<UP>(sG, sX, [&](int i, int c, float re, float im) {  // 2I - G X
+      sT[i * LD + c] = make_float2((i == c && c < (int)U ? 2.f : 0.f) - re, -im);
+    });
     __syncthreads();
-    cmatmul<UP>(sX, sT, acc);  // X (2I - G X)
+    cmm<UP>(sX, sT, store(sV));  // X (2I - G X)
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      xv[j] = acc[j];
-      sX[(r0 + RS * j) * LD + c] = acc[j];
-    }
-    __syncthreads();
+    float2* t = sX;
+    sX = sV;
+    sV = t;
   }
-  cmatmul<UP>(sX, sG, acc);  // X G
   float ss = 0.f;
-#pragma unroll
-  for (int j = 0; j < EPT; ++j) {
-    const int i = r0 + RS * j;
-    const float re = acc[j].x - ((i == c && cin) ? 1.f : 0.f);
-    ss += re * re + acc[j].y * acc[j].y;
-    if (cin && i < (int)U) X_out[o + i * U + c] = xv[j];
+  cmm<UP>(sX, sG, [&](int i, int c, float re, float im) {  // |X G - I|_F
+    const float d = re - (i == c && c < (int)U ? 1.f : 0.f);
+    ss += d * d + im * im;
+  });
+  for (int e = threadIdx.x; e < (int)(U * U); e += L::NTH) {
+    const int i = e / U, c = e % U;
+    X_out[o + e] = sX[i * LD + c];
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
@@ -348,7 +339,7 @@ hipError_t launch_refine_up(uint32_t U, uint32_t batch, const float2* G, const f
                             const float2* Xp, const float* corr, uint32_t corr_ld,
                             const float* scale, float2* X_out, float* res, int32_t* flag,
                             float bound, uint32_t iters, hipStream_t st) {
-  const size_t lds = (size_t)3 * UP * (UP + 1) * sizeof(float2);
+  const size_t lds = (size_t)4 * UP * (UP + 1) * sizeof(float2);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&refine_kernel<UP>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
