@@ -1,5 +1,6 @@
 // Mixed-precision zero-forcing weights (the precision form of the detection
-// path): per subcarrier group, one workgroup of 256 threads computes
+// path): per subcarrier group, one workgroup (one wave for up to 16 users,
+// else four) computes
 //   1. G = A A^H on the BF16 matrix cores with split operands (prec_core.h),
 //      FP32 accumulation, mirrored to an exactly Hermitian matrix;
 //   2. X0 = G^-1 by the FP32 sweep operator (zf_core.h), and the 1-norm
@@ -30,7 +31,9 @@ constexpr float kCondMax = 1e3f;  // kappa_1(G) above this takes the FP64 route
 
 namespace {
 
-constexpr int kMixThreads = 256;
+// Threads per group: one wave for up to 16 users, four waves otherwise.
+template <int UT>
+__host__ __device__ constexpr int mix_threads() { return UT == 1 ? 64 : 256; }
 
 template <int UT>
 struct ZfMixShared {
@@ -39,7 +42,7 @@ struct ZfMixShared {
   ZfShared<UT> sw;     // G, then -G^-1 (sweep); later X (refined inverse)
   float2 G0[UP * LDG];  // copy of G for the residual; later R; later split X
   float red[2];
-  float part[kMixThreads];  // column-norm shares
+  float part[mix_threads<UT>()];  // column-norm shares
 };
 
 
@@ -47,9 +50,9 @@ struct ZfMixShared {
 // root; the norm only feeds the condition estimate), result in *out after the
 // caller's next barrier. Every thread sums its share of a column into part,
 // then wave 0 adds the shares and takes the maximum. All threads call it.
-template <int UP, int LDG>
+template <int UP, int LDG, int NTH>
 __device__ __forceinline__ void col_norm1(const float2* M, uint32_t U, float* part, float* out) {
-  constexpr int RS = kMixThreads / UP, EPT = UP / RS;
+  constexpr int RS = NTH / UP, EPT = UP / RS;
   const int tid = threadIdx.x, c = tid % UP, r0 = tid / UP;
   float s = 0.f;
 #pragma unroll
@@ -75,10 +78,9 @@ __device__ __forceinline__ void col_norm1(const float2* M, uint32_t U, float* pa
 // FP64 sweep on M (UP x LDG, Hermitian, padded with identity) -> -M^-1, one
 // barrier per pivot; entries in registers, pivot row/column through prow/pcol
 // (2 x UP each). Returns 0 or k+1. Ends with a barrier.
-template <int UP, int LDG>
+template <int UP, int LDG, int NTH>
 __device__ int sweep_f64(double2* M, double2* prow, double2* pcol, double* diag,
                          uint32_t U) {
-  constexpr int NTH = kMixThreads;
   constexpr int EPT = UP * UP / NTH;
   constexpr int RS = NTH / UP;
   static_assert((UP * UP) % NTH == 0 && NTH % UP == 0, "sweep layout");
@@ -194,7 +196,7 @@ struct LowerTiles {
 template <int UT>
 __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_t U, uint32_t B,
                                            uint32_t* stage, float2* sG) {
-  constexpr int UP = 16 * UT, LDG = UP + 1, NTH = kMixThreads, NW = NTH / 64;
+  constexpr int UP = 16 * UT, LDG = UP + 1, NTH = mix_threads<UT>(), NW = NTH / 64;
   constexpr int PER = UP * kStageKC / NTH;  // staged elements per thread
   using T = LowerTiles<UT>;
   constexpr int TW = (T::N + NW - 1) / NW;  // tiles per wave (at most)
@@ -207,7 +209,7 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
 
   // all chunks of a batch of up to kStageBatch are loaded at once, so the
   // group's A is in flight together (one HBM round trip per batch)
-  constexpr int NB = UT >= 4 ? kStageBatch / 2 : kStageBatch;
+  constexpr int NB = UT == 2 ? kStageBatch : kStageBatch / 2;
   float2 pf[NB][PER];
   auto fetch = [&](uint32_t ch, float2 (&d)[PER]) {
 #pragma unroll
@@ -305,13 +307,13 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
 }
 
 template <int UT>
-__global__ __launch_bounds__(kMixThreads, UT >= 4 ? 1 : 2) void zf_mixed(const float2* __restrict__ in,
+__global__ __launch_bounds__(mix_threads<UT>(), UT >= 4 ? 1 : 2) void zf_mixed(const float2* __restrict__ in,
                                                         float2* w_out, uint32_t* ws_out,
                                                         int32_t* info, int32_t* route,
                                                         uint32_t U, uint32_t B) {
   constexpr int UP = 16 * UT;
   constexpr int LDG = UP + 1;
-  constexpr int NTH = kMixThreads;
+  constexpr int NTH = mix_threads<UT>();
   constexpr int NW = NTH / 64;
   __shared__ ZfMixShared<UT> sh;
   // Gram staging (gram_split), later the FP64 route: UP x LDG, prow/pcol, diag
@@ -343,13 +345,13 @@ __global__ __launch_bounds__(kMixThreads, UT >= 4 ? 1 : 2) void zf_mixed(const f
     sh.G0[i * LDG + c] = val;
   }
   __syncthreads();
-  col_norm1<UP, LDG>(sh.G0, U, sh.part, &sh.red[0]);
+  col_norm1<UP, LDG, NTH>(sh.G0, U, sh.part, &sh.red[0]);
 
   // ---- 2. FP32 sweep and condition number ------------------------------------
   int fail = zf_sweep<UT, NTH>(sh.sw, U);  // starts with the barrier it needs
   bool fp64 = fail != 0;
   if (!fp64) {
-    col_norm1<UP, LDG>(sG, U, sh.part, &sh.red[1]);
+    col_norm1<UP, LDG, NTH>(sG, U, sh.part, &sh.red[1]);
     __syncthreads();
     const float kappa = sh.red[0] * sh.red[1];
     fp64 = !(kappa <= kCondMax);
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(kMixThreads, UT >= 4 ? 1 : 2) void zf_mixed(const f
       }
     }
     __syncthreads();
-    fail = sweep_f64<UP, LDG>(M, prow, pcol, diag, U);
+    fail = sweep_f64<UP, LDG, NTH>(M, prow, pcol, diag, U);
     // W = G^-1 A in rounds of NTH weights, each read before any is written
     // (w may alias A).
     for (uint32_t base = 0; base < U * B; base += NTH) {
@@ -581,7 +583,7 @@ hipError_t launch_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((zf_mixed<UT>), dim3(batch), dim3(kMixThreads), lds, st, in, w_out, wp,
+  hipLaunchKernelGGL((zf_mixed<UT>), dim3(batch), dim3(mix_threads<UT>()), lds, st, in, w_out, wp,
                      info, route, U, B);
   return hipGetLastError();
 }
