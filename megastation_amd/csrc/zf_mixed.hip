@@ -157,20 +157,22 @@ __device__ __forceinline__ void store_pad(uint32_t* ws, uint32_t U, uint32_t Bp,
 }
 
 // ---- Gram on split BF16 products ------------------------------------------
-// Antenna chunks of kStageKC go through LDS once per group as split words:
+// Antenna chunks (stage_kc) go through LDS once per group as split words:
 // every thread loads its share of kStageBatch chunks at once (coalesced, the
 // whole batch in flight), then per chunk splits it once and writes the hi
-// and lo planes [user][antenna] (rows of kStageKCP words, 16-byte aligned),
+// and lo planes [user][antenna] (rows of 4 more words, 16-byte aligned),
 // double-buffered with one barrier per chunk. Output tiles are 8 users (rows, re/im
 // interleaved) x 16 users (columns); only tiles reaching the lower triangle
 // are computed (the caller mirrors), spread over the four waves.
-constexpr int kStageKC = 64;
-constexpr int kStageKCP = kStageKC + 4;
+// Chunk of antennas staged at a time: 32 for up to 16 users (small LDS
+// footprint, many groups per CU), else 64; rows padded by 4 words.
+template <int UT>
+__host__ __device__ constexpr int stage_kc() { return UT == 1 ? 32 : 64; }
 constexpr int kStageBatch = 4;
 
 template <int UT>
 __host__ __device__ constexpr size_t gram_stage_bytes() {
-  return (size_t)2 * 2 * 16 * UT * kStageKCP * sizeof(uint32_t);
+  return (size_t)2 * 2 * 16 * UT * (stage_kc<UT>() + 4) * sizeof(uint32_t);
 }
 
 template <int UT>
@@ -197,6 +199,7 @@ template <int UT>
 __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_t U, uint32_t B,
                                            uint32_t* stage, float2* sG) {
   constexpr int UP = 16 * UT, LDG = UP + 1, NTH = mix_threads<UT>(), NW = NTH / 64;
+  constexpr int kStageKC = stage_kc<UT>(), kStageKCP = kStageKC + 4;
   constexpr int PER = UP * kStageKC / NTH;  // staged elements per thread
   using T = LowerTiles<UT>;
   constexpr int TW = (T::N + NW - 1) / NW;  // tiles per wave (at most)
