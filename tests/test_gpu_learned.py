@@ -110,9 +110,10 @@ def test_gram_inv_matches_fp64(U, B):
     assert none is None and np.array_equal(host_c64(Gonly), host_c64(Gm))
 
 
-def test_refine_first_order_newton_schulz_and_judge():
-    print("case: refine", flush=True)
-    U, B, G = 32, 128, 12
+@pytest.mark.parametrize("U,B", [(32, 128), (12, 48), (40, 160)])
+def test_refine_first_order_newton_schulz_and_judge(U, B):
+    print(f"case: refine U={U} B={B}", flush=True)
+    G = 12
     ch = synth.TdlJakes(B, U, G * 12, 12, seed=2)
     a0, a1 = ch.slot(10), ch.slot(11)
     G0, X0 = learned.gram_inv(dev_c64(a0), U, B, G)
@@ -142,10 +143,11 @@ def test_refine_first_order_newton_schulz_and_judge():
     assert f.cpu().numpy().all()
 
 
-def test_finish_fallback_and_apply():
-    print("case: finish", flush=True)
+@pytest.mark.parametrize("U,B", [(32, 128), (12, 48), (40, 160)])
+def test_finish_fallback_and_apply(U, B):
+    print(f"case: finish U={U} B={B}", flush=True)
     t = torch()
-    U, B, G = 32, 128, 10
+    G = 10
     a = _groups(5, G, B, U)
     d_a = dev_c64(a)
     Gm, X = learned.gram_inv(d_a, U, B, G)
