@@ -170,9 +170,14 @@ template <int UT>
 __host__ __device__ constexpr int stage_kc() { return UT == 1 ? 32 : 64; }
 constexpr int kStageBatch = 4;
 
+// Staging buffers: one for one-wave groups (the smallest LDS footprint), two
+// (double-buffered) otherwise.
+template <int UT>
+__host__ __device__ constexpr int stage_bufs() { return UT == 1 ? 1 : 2; }
+
 template <int UT>
 __host__ __device__ constexpr size_t gram_stage_bytes() {
-  return (size_t)2 * 2 * 16 * UT * (stage_kc<UT>() + 4) * sizeof(uint32_t);
+  return (size_t)stage_bufs<UT>() * 2 * 16 * UT * (stage_kc<UT>() + 4) * sizeof(uint32_t);
 }
 
 template <int UT>
@@ -281,16 +286,20 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
       }
     }
   };
-  // chunks ch0 .. ch0+NB-1 per batch; chunk c of the batch uses buffer c & 1
+  // chunks ch0 .. ch0+NB-1 per batch; chunk c of the batch uses buffer
+  // c % stage_bufs (with one buffer, a barrier keeps each chunk's products
+  // ahead of the next chunk's writes)
+  constexpr int NBUF = stage_bufs<UT>();
   for (uint32_t ch0 = 0; ch0 < nch; ch0 += NB) {
 #pragma unroll
     for (int c = 0; c < NB; ++c) fetch(min(ch0 + c, nch - 1), pf[c]);
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
       if (ch0 + c >= nch) break;  // uniform
-      put(ch0 + c, pf[c], c & 1);
+      if (NBUF == 1 && c > 0) __syncthreads();
+      put(ch0 + c, pf[c], c % NBUF);
       __syncthreads();
-      products(c & 1);
+      products(c % NBUF);
     }
     __syncthreads();  // the last buffers are read before the next batch's writes
   }
@@ -310,7 +319,7 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
 }
 
 template <int UT>
-__global__ __launch_bounds__(mix_threads<UT>(), UT >= 4 ? 1 : 2) void zf_mixed(const float2* __restrict__ in,
+__global__ __launch_bounds__(mix_threads<UT>(), UT == 1 ? 16 : UT == 2 ? 2 : 1) void zf_mixed(const float2* __restrict__ in,
                                                         float2* w_out, uint32_t* ws_out,
                                                         int32_t* info, int32_t* route,
                                                         uint32_t U, uint32_t B) {
