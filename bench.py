@@ -186,45 +186,24 @@ def main():
             evs[1].record(stream)
             evs[2].record(stream)
 
-    # Slot-pipelined forms (single-slot workloads): the ZF of step i+1 goes
-    # out on a second stream before the equalize of step i, so the two run
-    # together; W triple-buffered; every dependency is an event (ZF(j) ->
-    # EQ(j), EQ(j) -> ZF(j+3) reusing its W).
-    zs = torch.cuda.Stream()
-    wbuf = [w, torch.empty_like(w), torch.empty_like(w)]
-    sbuf = [split, torch.empty_like(split), torch.empty_like(split)]
-    pipe_state = {"i": 0, "mixed": None, "zf_ev": [None] * 3, "eq_done": [None] * 3}
+    # Slot-pipelined forms (single-slot workloads): megastation_amd.pipeline.
+    # SlotPipeline issues the ZF of step i+1 on its own stream before the
+    # equalize of step i, so the two kernels run together (W triple-buffered,
+    # every dependency an event).
+    from megastation_amd.pipeline import SlotPipeline
+    pipes = {}
 
-    def zf_submit(j, mixed):
-        k = j % 3
-        done = pipe_state["eq_done"][k]
-        if done is not None:
-            zs.wait_event(done)
-        if mixed:
-            m.zf_split(csi, sbuf[k], B, U, slots * G, stream=zs)
-        else:
-            m.zf(csi, wbuf[k], B, U, slots * G, stream=zs)
-        ev = torch.cuda.Event()
-        ev.record(zs)
-        pipe_state["zf_ev"][k] = ev
+    def get_pipe(mixed):
+        if mixed not in pipes:
+            pipes[mixed] = SlotPipeline(
+                num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, mod_order=Q,
+                ue_stride=lay.ue_stride, cb_size=lay.cb_size, cb_stride=lay.cb_stride,
+                zeros=lay.zeros, valid_bits=lay.valid_bits,
+                form=m.FORM_MIXED if mixed else m.FORM_FP32, stream=stream)
+        return pipes[mixed]
 
     def step_pipe(mixed, evs=None):
-        if pipe_state["mixed"] != mixed:  # a new sequence: its first ZF
-            pipe_state.update(i=0, mixed=mixed, zf_ev=[None] * 3, eq_done=[None] * 3)
-            zf_submit(0, mixed)
-        i = pipe_state["i"]
-        pipe_state["i"] = i + 1
-        zf_submit(i + 1, mixed)  # the next step's ZF, ahead of this step's equalize
-        k = i % 3
-        stream.wait_event(pipe_state["zf_ev"][k])
-        if mixed:
-            m.equalize_slots(sbuf[k], y, llr, form=m.FORM_MIXED, stream=stream,
-                             **dict(eq_kw, csi_slot_stride=G * U * Bp))
-        else:
-            m.equalize_slots(wbuf[k], y, llr, stream=stream, **eq_kw)
-        eq_ev = torch.cuda.Event()
-        eq_ev.record(stream)
-        pipe_state["eq_done"][k] = eq_ev
+        get_pipe(mixed).push(csi, y, llr)
 
     if args.copy_calibration:
         src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
@@ -234,19 +213,21 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    def timed(step, pipe=False):
+    def timed(step, pipe=None):
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
-        if pipe:  # the timed region ends when both streams have drained
+        if pipe is not None:  # the timed region ends when both streams have drained
             t0 = torch.cuda.Event(enable_timing=True)
             t1 = torch.cuda.Event(enable_timing=True)
             t0.record(stream)
-            zs.wait_stream(stream)
-            for i in range(args.steps):
+            pipe.zs.wait_stream(stream)
+            for i in range(args.steps):  # K ZFs and K equalizes (the first one pending)
                 step()
-            stream.wait_stream(zs)
+            stream.wait_stream(pipe.zs)
             t1.record(stream)
+            torch.cuda.synchronize()
+            pipe.flush()
             torch.cuda.synchronize()
             return t0.elapsed_time(t1) / args.steps, None, None
         evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -273,7 +254,7 @@ def main():
     print(f"[bench] mixed form timed ({args.steps} steps)", file=sys.stderr, flush=True)
     if slots == 1:
         for name, mixed in (("plain_pipe", False), ("mixed_pipe", True)):
-            forms[name] = timed(lambda evs=None, mx=mixed: step_pipe(mx), pipe=True)
+            forms[name] = timed(lambda evs=None, mx=mixed: step_pipe(mx), pipe=get_pipe(mixed))
             print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
 
     symbols = U * N * S * slots
