@@ -138,9 +138,25 @@ def main():
     lay = synth.standalone_layout(N, Q)
     dev = torch.device("cuda:0")
     print(f"[bench] workload {args.workload}: generating inputs", file=sys.stderr, flush=True)
-    csi, y, x = gen_inputs(cfg)
-    w = torch.empty_like(csi)
-    llr = torch.empty(slots * S * U * lay.ue_stride, dtype=torch.int16, device=dev)
+    csi0, y0, x = gen_inputs(cfg)
+    # Successive steps rotate through NR distinct sets of input and output
+    # buffers, so that a one-slot working set (about 140 MB at 256x32, inside
+    # the 256 MB Infinity Cache) is not served from that cache step after step:
+    # four sets for one-slot workloads, one for multi-slot ones (GBs per step).
+    NR = 4 if slots == 1 else 1
+    csi_r = [csi0] + [csi0.clone() for _ in range(NR - 1)]
+    y_r = [y0] + [y0.clone() for _ in range(NR - 1)]
+    w_r = [torch.empty_like(csi0) for _ in range(NR)]
+    llr_r = [torch.empty(slots * S * U * lay.ue_stride, dtype=torch.int16, device=dev)
+             for _ in range(NR)]
+    csi, y, w, llr = csi0, y0, w_r[0], llr_r[0]
+    ring = [0]
+
+    def nxt():
+        """Buffer set of the next step (csi, y, w, llr, index)."""
+        j = ring[0] % NR
+        ring[0] += 1
+        return csi_r[j], y_r[j], w_r[j], llr_r[j], j
     stream = torch.cuda.current_stream()
 
     eq_kw = dict(num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S,
@@ -152,44 +168,50 @@ def main():
 
     def step_plain(evs=None):
         """Drop-in form: ZF kernel (W to HBM) then the equalize kernel."""
+        c_, y_, w_, l_, _ = nxt()
         if evs:
             evs[0].record(stream)
-        m.zf(csi, w, B, U, slots * G, stream=stream)
+        m.zf(c_, w_, B, U, slots * G, stream=stream)
         if evs:
             evs[1].record(stream)
-        m.equalize_slots(w, y, llr, stream=stream, **eq_kw)
+        m.equalize_slots(w_, y_, l_, stream=stream, **eq_kw)
         if evs:
             evs[2].record(stream)
 
     Bp = (B + 15) // 16 * 16
-    split = torch.empty(slots * G * U * Bp, dtype=torch.int64, device=dev)
+    split_r = [torch.empty(slots * G * U * Bp, dtype=torch.int64, device=dev)
+               for _ in range(NR)]
+    split = split_r[0]
 
     def step_mixed(evs=None):
         """Mixed form: split-BF16 matrix-core ZF (split weight words to HBM),
         then the split-BF16 equalize kernel."""
+        c_, y_, _, l_, j = nxt()
         if evs:
             evs[0].record(stream)
-        m.zf_split(csi, split, B, U, slots * G, stream=stream)
+        m.zf_split(c_, split_r[j], B, U, slots * G, stream=stream)
         if evs:
             evs[1].record(stream)
-        m.equalize_slots(split, y, llr, form=m.FORM_MIXED, stream=stream,
+        m.equalize_slots(split_r[j], y_, l_, form=m.FORM_MIXED, stream=stream,
                          **dict(eq_kw, csi_slot_stride=G * U * Bp))
         if evs:
             evs[2].record(stream)
 
     def step_fused(evs=None):
         """Fused form: ZF + equalize + demod in one kernel, W kept on chip."""
+        c_, y_, _, l_, _ = nxt()
         if evs:
             evs[0].record(stream)
-        m.detect_slots(csi, y, llr, stream=stream, **eq_kw)
+        m.detect_slots(c_, y_, l_, stream=stream, **eq_kw)
         if evs:
             evs[1].record(stream)
             evs[2].record(stream)
 
-    # Slot-pipelined forms (single-slot workloads): megastation_amd.pipeline.
-    # SlotPipeline issues the ZF of step i+1 on its own stream before the
-    # equalize of step i, so the two kernels run together (W triple-buffered,
-    # every dependency an event).
+    # Slot-pipelined forms (single-slot workloads): the library's slot pipeline
+    # (mega_pipeline_*, through megastation_amd.pipeline.SlotPipeline) issues
+    # the ZF of step i+1 on its own stream before the equalize of step i, so
+    # the two kernels run together (W triple-buffered, every dependency an
+    # event, one C call per step).
     from megastation_amd.pipeline import SlotPipeline
     pipes = {}
 
@@ -203,7 +225,35 @@ def main():
         return pipes[mixed]
 
     def step_pipe(mixed, evs=None):
-        get_pipe(mixed).push(csi, y, llr)
+        c_, y_, _, l_, _ = nxt()
+        get_pipe(mixed).push(c_, y_, l_)
+
+    # Graph forms (single-slot workloads): K consecutive slots (K = --steps,
+    # buffer sets rotating as above) pipelined as SlotPipeline would and
+    # captured into one graph (mega_pipeline_graph_*); one launch runs the K
+    # steps with no host work between them.
+    from megastation_amd.pipeline import PipelineGraph
+
+    def make_graph(mixed):
+        return PipelineGraph(
+            [(csi_r[i % NR], y_r[i % NR], llr_r[i % NR]) for i in range(args.steps)],
+            num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, mod_order=Q,
+            ue_stride=lay.ue_stride, cb_size=lay.cb_size, cb_stride=lay.cb_stride,
+            zeros=lay.zeros, valid_bits=lay.valid_bits,
+            form=m.FORM_MIXED if mixed else m.FORM_FP32)
+
+    def timed_graph(gr):
+        """W warm-up launches, then one timed launch of exactly K steps."""
+        for _ in range(max(1, args.warmup)):
+            gr.launch(stream)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        gr.launch(stream)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / args.steps, None, None
 
     if args.copy_calibration:
         src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
@@ -212,6 +262,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    host_ms = {}  # host time to issue one pipelined step (launches, waits, records)
 
     def timed(step, pipe=None):
         for _ in range(args.warmup):
@@ -222,8 +274,10 @@ def main():
             t1 = torch.cuda.Event(enable_timing=True)
             t0.record(stream)
             pipe.zs.wait_stream(stream)
+            h0 = time.perf_counter()
             for i in range(args.steps):  # K ZFs and K equalizes (the first one pending)
                 step()
+            host_ms[pipe] = (time.perf_counter() - h0) * 1e3 / args.steps
             stream.wait_stream(pipe.zs)
             t1.record(stream)
             torch.cuda.synchronize()
@@ -256,6 +310,11 @@ def main():
         for name, mixed in (("plain_pipe", False), ("mixed_pipe", True)):
             forms[name] = timed(lambda evs=None, mx=mixed: step_pipe(mx), pipe=get_pipe(mixed))
             print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
+        for name, mixed in (("plain_graph", False), ("mixed_graph", True)):
+            gr = make_graph(mixed)
+            forms[name] = timed_graph(gr)
+            gr.close()
+            print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
 
     symbols = U * N * S * slots
     io_bytes = 8 * B * N * S * slots + 8 * U * B * G * slots + 2 * U * Q * N * S * slots
@@ -270,7 +329,7 @@ def main():
     x0 = x[0].cpu().numpy()
     evm = {}
     for name in forms:
-        if forms[name] is None or name.endswith("_pipe"):
+        if forms[name] is None or name.endswith(("_pipe", "_graph")):
             continue
         xh.zero_()
         if name == "fused":
@@ -285,8 +344,8 @@ def main():
         torch.cuda.synchronize()
         evm[name] = synth.evm_db(xh.cpu().numpy().reshape(S, U, N), x0)
     for name in forms:
-        if name.endswith("_pipe"):  # same kernels and inputs as the sequential form
-            evm[name] = evm[name[:-5]]
+        if name.endswith(("_pipe", "_graph")):  # same kernels and inputs as the sequential form
+            evm[name] = evm[name.rsplit("_", 1)[0]]
     evm64 = fp64_evm(cfg, csi, y, x0)
     ok = {k: abs(v - evm64) < 0.1 for k, v in evm.items()}
 
@@ -297,7 +356,7 @@ def main():
     if head_form == "fused":
         dom_kernel, dom_ms, dom_prefix = ("detect_fused (ZF + equalize + demod)",
                                           forms["fused"][1], "detect_fused")
-    elif head_form in ("plain", "plain_pipe"):
+    elif head_form in ("plain", "plain_pipe", "plain_graph"):
         dom_kernel, dom_ms, dom_prefix = "equalize_demod", forms["plain"][2], "equalize_"
     else:
         dom_kernel, dom_ms, dom_prefix = ("equalize_demod (split BF16)", forms["mixed"][2],
@@ -317,7 +376,11 @@ def main():
              "evm_db": evm[k], "evm_delta_db": evm[k] - evm64, "evm_ok": ok[k]}
         if k.endswith("_pipe"):
             e["schedule"] = ("ZF of step i+1 on a second stream, issued before the "
-                             "equalize of step i; W triple-buffered")
+                             "equalize of step i; W triple-buffered; one C call per step")
+            e["host_ms_per_step"] = host_ms.get(get_pipe(k == "mixed_pipe"))
+        elif k.endswith("_graph"):
+            e["schedule"] = (f"the {k[:-6]}_pipe schedule of {args.steps} consecutive steps "
+                             "captured into one graph, one launch")
         elif k != "fused":
             e.update({"zf_ms": t1, "equalize_ms": t2,
                       "zf_tflops": zf_flops / (t1 * 1e-3) / 1e12,

@@ -226,6 +226,57 @@ int mega_batch_run(mega_batch_t batch, mega_stream_t stream);
 const char *mega_batch_plan(mega_batch_t batch);
 int mega_batch_destroy(mega_batch_t batch);
 
+/* ---- Slot pipeline: ZF of the next slot beside the current equalize ---------
+ * Consecutive slots of one cell, one call per slot. push(slot i) issues the
+ * ZF of slot i on the pipeline's own stream, then the equalize + demod of
+ * slot i-1 on `stream` (bound at create), so the two kernels run together.
+ * Weights live in `depth` (>= 2) device buffers the pipeline owns; every
+ * dependency is an event: the equalize of a slot waits for its ZF, and a
+ * ZF waits for the equalize that last read its weight buffer. csi is read on
+ * the ZF stream (mega_pipeline_zf_stream): produce it there, or make that
+ * stream wait for its producer; y is read on `stream`. Results equal
+ * mega_zf (FP32 form) or
+ * mega_zf_split (mixed form) followed by mega_equalize_slots of each slot,
+ * bit for bit. No single reference counterpart: it schedules the
+ * Beamform::beamform_uplink + Equalize::equalize calls of consecutive frames
+ * (process.cc:172-185, gpu_engine.cc:289-343). `shape` gives the cell:
+ * num_ue .. valid_bits, form (MEGA_FORM_FP32 or MEGA_FORM_MIXED) and the y /
+ * llr symbol strides; its buffers, num_slots and slot strides are ignored.
+ * A slot's csi, y, llr and xhat (optional) must stay valid until its
+ * equalize has run on `stream` (issued by the next push or by flush). One
+ * thread per handle, on the device current at create; destroy waits for the
+ * pipeline's work. Group failures are not reported (use mega_zf for that). */
+typedef struct mega_pipeline *mega_pipeline_t;
+
+int mega_pipeline_create(const mega_equalize_desc *shape, uint32_t depth,
+                         mega_stream_t stream, mega_pipeline_t *out);
+int mega_pipeline_push(mega_pipeline_t pipe, const void *csi, const void *y,
+                       void *llr, void *xhat);
+/* Issues the equalize of the last pushed slot. */
+int mega_pipeline_flush(mega_pipeline_t pipe);
+/* The pipeline's ZF stream, where csi is read (to order its producer). */
+mega_stream_t mega_pipeline_zf_stream(mega_pipeline_t pipe);
+int mega_pipeline_destroy(mega_pipeline_t pipe);
+
+/* n consecutive slots of one cell with their buffers bound (csi[i], y[i],
+ * llr[i], and xhat[i] if xhat is not NULL), scheduled as n pushes and a
+ * flush of the slot pipeline above and captured once into a graph (weights
+ * in three buffers the handle owns). mega_pipeline_graph_launch runs all n
+ * slots with one launch on `stream`, after the work issued there before it;
+ * results equal those of the pipeline (and of the sequential calls) bit for
+ * bit. The host issues nothing per slot, so the two streams' kernels follow
+ * each other without host gaps. Launches of one handle must not overlap
+ * (they share the weight buffers); destroy waits for the last launch. No
+ * single reference counterpart (see the slot pipeline). */
+typedef struct mega_pipeline_graph *mega_pipeline_graph_t;
+
+int mega_pipeline_graph_create(const mega_equalize_desc *shape, uint32_t n,
+                               const void *const *csi, const void *const *y,
+                               void *const *llr, void *const *xhat,
+                               mega_pipeline_graph_t *out);
+int mega_pipeline_graph_launch(mega_pipeline_graph_t graph, mega_stream_t stream);
+int mega_pipeline_graph_destroy(mega_pipeline_graph_t graph);
+
 /* ---- Learned form: network warm start for the ZF inverse --------------------
  * Building blocks of the learned ZF of a slot sequence (no reference
  * counterpart; the reference recomputes every slot by Cholesky). Matrices

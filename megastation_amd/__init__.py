@@ -122,6 +122,16 @@ def lib() -> ctypes.CDLL:
         "mega_batch_run": ([P, P], i32),
         "mega_batch_plan": ([P], ctypes.c_char_p),
         "mega_batch_destroy": ([P], i32),
+        "mega_pipeline_create": ([ctypes.POINTER(EqualizeDesc), u32, P, ctypes.POINTER(P)], i32),
+        "mega_pipeline_push": ([P, P, P, P, P], i32),
+        "mega_pipeline_flush": ([P], i32),
+        "mega_pipeline_zf_stream": ([P], P),
+        "mega_pipeline_destroy": ([P], i32),
+        "mega_pipeline_graph_create": ([ctypes.POINTER(EqualizeDesc), u32, ctypes.POINTER(P),
+                                        ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
+                                        ctypes.POINTER(P)], i32),
+        "mega_pipeline_graph_launch": ([P, P], i32),
+        "mega_pipeline_graph_destroy": ([P], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -50,8 +50,5 @@ hipError_t launch_reset_counters(uint32_t* counters, uint32_t n, hipStream_t st)
 size_t zf_class_lds(uint32_t UT, uint32_t U, uint32_t B, bool a_lds);
 size_t zf_class_static_lds(uint32_t UT);
 
-// Records a message for mega_last_error and returns code (capi.cpp).
-int set_error(int code, const std::string& msg);
-
 }  // namespace detail
 }  // namespace mega

@@ -173,7 +173,8 @@ int mega_zf(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
 }
 
 static int eq_args_checked(const mega_equalize_desc* d, mega::detail::EqualizeArgs& a,
-                           bool& empty, const char* who, bool allow_mixed = false) {
+                           bool& empty, const char* who, bool allow_mixed = false,
+                           bool need_buffers = true) {
   empty = false;
   if (!d) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": desc is NULL");
   if (!valid_q(d->mod_order))
@@ -187,7 +188,7 @@ static int eq_args_checked(const mega_equalize_desc* d, mega::detail::EqualizeAr
     empty = true;
     return MEGA_OK;
   }
-  if (!d->csi || !d->y || !d->llr)
+  if (need_buffers && (!d->csi || !d->y || !d->llr))
     return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
   if (d->sc_group == 0 || d->cb_size == 0)
     return fail(MEGA_ERR_INVALID_ARG, "sc_group and cb_size must be > 0");
@@ -219,6 +220,16 @@ static int eq_args_checked(const mega_equalize_desc* d, mega::detail::EqualizeAr
   a.llr_slot = d->llr_slot_stride;
   return MEGA_OK;
 }
+
+}  // extern "C"
+
+int mega::detail::equalize_args_checked(const mega_equalize_desc* d, EqualizeArgs& a,
+                                        bool& empty, const char* who, bool allow_mixed,
+                                        bool need_buffers) {
+  return eq_args_checked(d, a, empty, who, allow_mixed, need_buffers);
+}
+
+extern "C" {
 
 int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
   mega::detail::EqualizeArgs a;

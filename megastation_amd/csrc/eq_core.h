@@ -22,11 +22,22 @@
 // tile).
 #pragma once
 
+#include <type_traits>
+
 #include "mega/demod_core.h"
 #include "internal.h"
 
 namespace mega {
 namespace detail {
+
+// f(integral_constant<int, I>) for I = B .. E-1, unrolled at compile time.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
+
+#include "../../include/megadetect.h"
 
 namespace mega {
 namespace detail {
@@ -75,6 +78,15 @@ hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
 
 // Limits of the kernels (checked by the C ABI before launching).
 constexpr uint32_t kMaxUsers = 64;
+
+// Records a message for mega_last_error and returns code (capi.cpp).
+int set_error(int code, const std::string& msg);
+
+// Checks an equalize descriptor and fills a from it (capi.cpp): empty is set
+// for a zero-sized launch; allow_mixed admits MEGA_FORM_MIXED; need_buffers
+// requires csi, y and llr.
+int equalize_args_checked(const mega_equalize_desc* d, EqualizeArgs& a, bool& empty,
+                          const char* who, bool allow_mixed, bool need_buffers);
 
 }  // namespace detail
 }  // namespace mega

@@ -33,23 +33,12 @@
 //      them before storing, so writing W over A (in place) is safe.
 #pragma once
 
-#include <type_traits>
-
 #include "eq_core.h"
 
 namespace mega {
 namespace detail {
 
 constexpr float kPivotTol = 1.1920929e-7f;  // 2^-23, times U
-
-// f(integral_constant<int, I>) for I = B .. E-1, unrolled at compile time.
-template <int B, int E, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
 
 template <int UT>
 struct ZfShared {

@@ -1,5 +1,5 @@
-"""Slot pipeline of one cell: ZF of slot i+1 overlapped with the equalize of
-slot i on a second stream.
+"""Slot pipeline of one cell (mega_pipeline_*): ZF of slot i+1 overlapped with
+the equalize of slot i on a second stream.
 
 The reference's baseband pipeline runs the beamformer of a frame, then the
 equalize calls of its symbols, one frame after another on its streams
@@ -7,18 +7,46 @@ equalize calls of its symbols, one frame after another on its streams
 consecutive slots of a cell overlap: push(slot i) issues the ZF of slot i on
 the pipeline's ZF stream, *then* the equalize of slot i-1 on the caller's
 stream, so that both kernels are resident together (issued the other way
-round, the ZF waits for the equalize waves to drain). The weights are
-triple-buffered; every dependency is an event: ZF(i) -> equalize(i), and
-equalize(i) -> ZF(i+3), which reuses its weight buffer. Results are those of
-the sequential path (same kernels on the same inputs), bit for bit.
+round, the ZF waits for the equalize waves to drain). The scheduling lives in
+the library (one C call per slot; streams and events created once); weights
+are kept in `depth` buffers it owns. Results are those of the sequential path
+(same kernels on the same inputs), bit for bit.
+
+PipelineGraph (mega_pipeline_graph_*) binds n consecutive slots' buffers and
+captures their pipelined schedule into one graph: one launch runs all n
+slots, with no host work per slot.
 
 Both forms: FORM_FP32 (mega_zf + mega_equalize_slots) and FORM_MIXED
 (mega_zf_split + the split-BF16 equalize). One slot per push; csi is
-[G][B][U] complex64 (A = H^H), y [S][B][N] complex64, llr the slot's LLR rows.
+[G][B][U] complex64 (A = H^H), read on the pipeline's ZF stream `zs` (produce
+it there, or make `zs` wait for its producer); y [S][B][N] complex64 and llr,
+the slot's LLR rows ([S][U][ue_stride] halves), on the caller's stream.
 """
 from __future__ import annotations
 
-from . import FORM_FP32, FORM_MIXED, equalize_slots, zf, zf_split
+import collections
+import ctypes
+
+from . import FORM_FP32, FORM_MIXED, EqualizeDesc, _check, _ptr, _stream, lib
+
+
+def _shape_desc(num_ue, num_bs, num_sc, sc_group, num_syms, mod_order, ue_stride, cb_size,
+                cb_stride, zeros, valid_bits, form):
+    d = EqualizeDesc()
+    d.num_ue, d.num_bs, d.num_sc, d.sc_group = num_ue, num_bs, num_sc, sc_group
+    d.num_syms, d.num_slots, d.mod_order, d.form = num_syms, 1, mod_order, form
+    d.ue_stride, d.cb_size, d.cb_stride = ue_stride, cb_size, cb_stride
+    d.zeros, d.valid_bits = zeros, valid_bits
+    d.y_sym_stride = num_bs * num_sc
+    d.llr_sym_stride = num_ue * ue_stride
+    return d
+
+
+def _check_form(form, depth=3):
+    if form not in (FORM_FP32, FORM_MIXED):
+        raise ValueError(f"unsupported form {form}")
+    if depth < 2:
+        raise ValueError("depth must be at least 2 (the ZF runs one slot ahead)")
 
 
 class SlotPipeline:
@@ -26,73 +54,83 @@ class SlotPipeline:
                  cb_size, cb_stride, zeros, valid_bits, form: int = FORM_MIXED,
                  stream=None, depth: int = 3):
         import torch
-        if form not in (FORM_FP32, FORM_MIXED):
-            raise ValueError(f"unsupported form {form}")
-        if depth < 2:
-            raise ValueError("depth must be at least 2 (the ZF runs one slot ahead)")
-        self.U, self.B, self.N, self.sg, self.S = num_ue, num_bs, num_sc, sc_group, num_syms
-        self.G = (num_sc + sc_group - 1) // sc_group
-        self.form = form
-        self.Bp = (num_bs + 15) // 16 * 16
+        _check_form(form, depth)
         self.stream = stream if stream is not None else torch.cuda.current_stream()
-        self.zs = torch.cuda.Stream()
-        dev = torch.device("cuda", torch.cuda.current_device())
-        if form == FORM_MIXED:
-            self.wbuf = [torch.empty(self.G * self.U * self.Bp, dtype=torch.int64, device=dev)
-                         for _ in range(depth)]
-        else:
-            self.wbuf = [torch.empty(self.G * self.B * self.U, dtype=torch.complex64, device=dev)
-                         for _ in range(depth)]
-        self.eq_kw = dict(num_ue=num_ue, num_bs=num_bs, num_sc=num_sc, sc_group=sc_group,
-                          num_syms=num_syms, num_slots=1, mod_order=mod_order,
-                          ue_stride=ue_stride, cb_size=cb_size, cb_stride=cb_stride,
-                          zeros=zeros, valid_bits=valid_bits,
-                          csi_slot_stride=(self.G * self.U * self.Bp if form == FORM_MIXED
-                                           else self.G * self.B * self.U),
-                          y_sym_stride=num_bs * num_sc, y_slot_stride=num_syms * num_bs * num_sc,
-                          llr_sym_stride=num_ue * ue_stride,
-                          llr_slot_stride=num_syms * num_ue * ue_stride)
+        d = _shape_desc(num_ue, num_bs, num_sc, sc_group, num_syms, mod_order, ue_stride,
+                        cb_size, cb_stride, zeros, valid_bits, form)
+        self._h = ctypes.c_void_p()
+        _check(lib().mega_pipeline_create(ctypes.byref(d), depth, _stream(self.stream),
+                                          ctypes.byref(self._h)))
+        self.zs = torch.cuda.ExternalStream(lib().mega_pipeline_zf_stream(self._h))
         self.depth = depth
-        self.i = 0             # slots pushed
-        self.zf_ev = [None] * depth
-        self.eq_done = [None] * depth
-        self.pending = None    # (y, llr, xhat) of the slot whose equalize is not issued
-
-    def _zf(self, csi, k):
-        import torch
-        if self.eq_done[k] is not None:
-            self.zs.wait_event(self.eq_done[k])
-        n = self.G
-        if self.form == FORM_MIXED:
-            zf_split(csi, self.wbuf[k], self.B, self.U, n, stream=self.zs)
-        else:
-            zf(csi, self.wbuf[k], self.B, self.U, n, stream=self.zs)
-        ev = torch.cuda.Event()
-        ev.record(self.zs)
-        self.zf_ev[k] = ev
-
-    def _equalize(self, k, y, llr, xhat):
-        import torch
-        self.stream.wait_event(self.zf_ev[k])
-        equalize_slots(self.wbuf[k], y, llr, xhat=xhat, form=self.form, stream=self.stream,
-                       **self.eq_kw)
-        ev = torch.cuda.Event()
-        ev.record(self.stream)
-        self.eq_done[k] = ev
+        # the tensors of the last depth+1 slots stay referenced: by then a
+        # slot's equalize (after its ZF) is issued on the caller's stream
+        self._keep = collections.deque(maxlen=depth + 1)
 
     def push(self, csi, y, llr, xhat=None):
         """Issue the ZF of this slot, then the equalize of the previous one.
         The buffers of a slot must stay valid until its equalize has run
         (after the next push or flush, on the caller's stream)."""
-        k = self.i % self.depth
-        self._zf(csi, k)
-        if self.pending is not None:
-            self._equalize((self.i - 1) % self.depth, *self.pending)
-        self.pending = (y, llr, xhat)
-        self.i += 1
+        self._keep.append((csi, y, llr, xhat))
+        _check(lib().mega_pipeline_push(self._h, _ptr(csi), _ptr(y), _ptr(llr), _ptr(xhat)))
 
     def flush(self):
         """Issue the equalize of the last pushed slot."""
-        if self.pending is not None:
-            self._equalize((self.i - 1) % self.depth, *self.pending)
-            self.pending = None
+        _check(lib().mega_pipeline_flush(self._h))
+
+    def close(self):
+        """Wait for the pipeline's work and free its buffers, streams and events."""
+        if self._h is not None and self._h.value:
+            _check(lib().mega_pipeline_destroy(self._h))
+        self._h = None
+        self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PipelineGraph:
+    """n consecutive slots of one cell, buffers bound, pipelined as SlotPipeline
+    would and captured into one graph; launch() runs them all. slots: a list
+    of (csi, y, llr) or (csi, y, llr, xhat) tuples of device tensors (xhat
+    given for all slots or none), which stay referenced by the handle."""
+
+    def __init__(self, slots, *, num_ue, num_bs, num_sc, sc_group, num_syms, mod_order,
+                 ue_stride, cb_size, cb_stride, zeros, valid_bits, form: int = FORM_MIXED):
+        _check_form(form)
+        if not slots:
+            raise ValueError("no slots")
+        with_x = len(slots[0]) > 3 and slots[0][3] is not None
+        n = len(slots)
+        arr = ctypes.c_void_p * n
+        self._keep = [tuple(s_) for s_ in slots]
+        csi = arr(*(_ptr(s_[0]) for s_ in slots))
+        y = arr(*(_ptr(s_[1]) for s_ in slots))
+        llr = arr(*(_ptr(s_[2]) for s_ in slots))
+        xh = arr(*(_ptr(s_[3]) for s_ in slots)) if with_x else None
+        d = _shape_desc(num_ue, num_bs, num_sc, sc_group, num_syms, mod_order, ue_stride,
+                        cb_size, cb_stride, zeros, valid_bits, form)
+        self._h = ctypes.c_void_p()
+        _check(lib().mega_pipeline_graph_create(ctypes.byref(d), n, csi, y, llr, xh,
+                                                ctypes.byref(self._h)))
+        self.n = n
+
+    def launch(self, stream=None):
+        """All n slots, one launch on stream (default: the current stream)."""
+        _check(lib().mega_pipeline_graph_launch(self._h, _stream(stream)))
+
+    def close(self):
+        """Wait for the last launch and free the graph and its buffers."""
+        if self._h is not None and self._h.value:
+            _check(lib().mega_pipeline_graph_destroy(self._h))
+        self._h = None
+        self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -49,6 +49,45 @@ def test_bad_beamform_arguments():
     assert L.mega_beamform_destroy(None) == m.MEGA_OK
 
 
+def test_bad_pipeline_arguments():
+    L = m.lib()
+    d = m.EqualizeDesc()
+    d.num_ue, d.num_bs, d.num_sc, d.sc_group, d.num_syms, d.num_slots = 4, 8, 16, 4, 1, 1
+    d.mod_order, d.form, d.ue_stride, d.cb_size, d.cb_stride = 4, m.FORM_MIXED, 64, 64, 64
+    d.valid_bits = 64
+    h = ctypes.c_void_p()
+    assert L.mega_pipeline_create(ctypes.byref(d), 1, None, ctypes.byref(h)) == \
+        m.MEGA_ERR_INVALID_ARG
+    assert "depth" in m.last_error()
+    assert L.mega_pipeline_create(ctypes.byref(d), 3, None, None) == m.MEGA_ERR_INVALID_ARG
+    d.form = 7
+    assert L.mega_pipeline_create(ctypes.byref(d), 3, None, ctypes.byref(h)) == \
+        m.MEGA_ERR_UNSUPPORTED
+    d.form, d.mod_order = m.FORM_FP32, 5
+    assert L.mega_pipeline_create(ctypes.byref(d), 3, None, ctypes.byref(h)) == \
+        m.MEGA_ERR_UNSUPPORTED
+    d.mod_order, d.num_sc = 4, 0
+    assert L.mega_pipeline_create(ctypes.byref(d), 3, None, ctypes.byref(h)) == \
+        m.MEGA_ERR_INVALID_ARG
+    assert h.value is None
+    assert L.mega_pipeline_push(None, None, None, None, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_pipeline_flush(None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_pipeline_zf_stream(None) is None
+    assert L.mega_pipeline_destroy(None) == m.MEGA_OK
+    d.num_sc = 16
+    arr = (ctypes.c_void_p * 2)(1, 1)
+    none = (ctypes.c_void_p * 2)(1, None)
+    assert L.mega_pipeline_graph_create(ctypes.byref(d), 0, arr, arr, arr, None,
+                                        ctypes.byref(h)) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_pipeline_graph_create(ctypes.byref(d), 2, arr, arr, None, None,
+                                        ctypes.byref(h)) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_pipeline_graph_create(ctypes.byref(d), 2, arr, none, arr, None,
+                                        ctypes.byref(h)) == m.MEGA_ERR_INVALID_ARG
+    assert "slot 1" in m.last_error()
+    assert L.mega_pipeline_graph_launch(None, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_pipeline_graph_destroy(None) == m.MEGA_OK
+
+
 def test_equalize_desc_layout_matches_header():
     # offsets of the C struct on LP64 (natural alignment)
     D = m.EqualizeDesc

@@ -177,6 +177,30 @@ def test_detect_mixed_matches_fp64_oracle(U, B, N, sg, Q, channel, snr, layout, 
             assert abs(e_mix - e_64) < 0.1, (e_mix, e_64)
 
 
+@pytest.mark.parametrize("U,B,N,ks", [(32, 256, 600, 1), (32, 256, 600, 2), (32, 256, 600, 4),
+                                      (64, 128, 300, 2), (16, 256, 250, 4)])
+def test_mixed_equalize_antenna_split(monkeypatch, U, B, N, ks):
+    """The streamed mixed equalize with the antennas of a unit split over
+    ks waves of a workgroup (partial sums met in LDS), every split the
+    launcher can take, forced by MEGA_EQ_XKS; a split changes the FP32
+    summation order only, so each is held to the FP64 oracle's bar
+    (partial last group included: N % sg != 0)."""
+    monkeypatch.setenv("MEGA_EQ_XKS", str(ks))
+    print(f"case: mixed equalize antenna split ks={ks} U={U} B={B} N={N}", flush=True)
+    sg, Q, S, slots = 12, 6, 2, 1
+    sl = make_slots(U + B + N + ks, slots, U, B, N, sg, S, Q, 25.0, "iid")
+    lay = synth.ldpc_uplink_layout(N, Q)
+    G = (N + sg - 1) // sg
+    csi = np.stack([s.csi for s in sl])
+    y = np.stack([s.y for s in sl])
+    llr, xh, info, route = _detect_mixed(csi, y, U, B, N, sg, Q, lay, slots, S)
+    assert not info.any() and not route.any()
+    w64, _, _ = oracle.zf(sl[0].csi, U, B, G, "f64")
+    for t_ in range(S):
+        llr64, x64 = oracle_symbol_f64(w64, sl[0].y[t_], U, B, N, sg, Q, lay)
+        check_symbol(llr[0, t_], xh[0, t_], llr64, x64, N, sg, Q, lay)
+
+
 def test_mixed_error_grows_with_condition_number():
     """Split-route accuracy on less well-conditioned groups (32 users on 64
     correlated antennas, kappa_1 up to about 370, below the FP64 route's
