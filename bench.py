@@ -13,14 +13,15 @@ matrix-core ZF and equalize). Single-slot workloads (--workload c3_256x32)
 also time both forms slot-pipelined: ZF of step i+1 on a second stream, issued
 before the equalize of step i, so the two run together (W triple-buffered,
 event-ordered). The headline is
-the fastest form whose EVM on slot 0 is within 0.1 dB of the FP64 oracle's. Synthetic data generated on the
+the fastest form whose EVM on slot 0 is within 0.1 dB of FP64 zero-forcing's. Synthetic data generated on the
 GPU with a seeded Philox generator.
 
 Prints one JSON line: value = detected symbols/s = U*N*S*slots / step time;
 roofline of the dominant kernel (equalize/demod, HBM-bound) from HIP events
 on the launch stream; the CPU baseline (the oracle's OpenMP restatement of
 the same operator math) timed on a bounded sample on the host cores; EVM of
-the GPU's equalized symbols against the FP64 oracle's on a sample slot.
+the GPU's equalized symbols next to FP64 zero-forcing's (numpy complex128)
+on a sample slot.
 """
 import argparse
 import json
@@ -110,7 +111,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2_64x16_200slots",
-                    choices=sorted(WORKLOADS) + ["c4_40cells", "c5_learned"])
+                    choices=sorted(WORKLOADS) + ["c4_40cells", "c5_learned",
+                                                 "d3_precode_256x32"])
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -122,6 +124,8 @@ def main():
         return main_batch(args)
     if args.workload == "c5_learned" and args.gpus == 1:
         return main_learned(args)
+    if args.workload == "d3_precode_256x32" and args.gpus == 1:
+        return main_precode(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the detection path shards by subcarrier group with no exchange; "
@@ -322,7 +326,7 @@ def main():
     eq_flops = 8 * U * B * N * S * slots
 
     # ---- EVM of every form on slot 0 (x_hat side output, untimed) against the
-    # transmitted symbols, next to the FP64 oracle's ----------------------------
+    # transmitted symbols, next to FP64 zero-forcing's ---------------------------
     xh = torch.zeros(S * U * N, dtype=torch.complex64, device=dev)
     kw1 = dict(eq_kw, num_slots=1)
     llr1 = torch.empty(S * U * lay.ue_stride, dtype=torch.int16, device=dev)
@@ -510,7 +514,7 @@ def main_learned(args):
 
     ms_plain, _, _ = run("plain")
     ms_learned, fb_rate, z = run("learned")
-    # EVM of the last slot's learned W against the FP64 oracle's
+    # EVM of the last slot's learned W against FP64 zero-forcing's
     A, y, x = slots[-1]
     xh = torch.zeros(S * U * N, dtype=torch.complex64, device=dev)
     W, _ = z.step(A)  # one more step of the same slot sequence state
@@ -542,6 +546,98 @@ def main_learned(args):
         "evm": {"gpu_db": evm_gpu, "fp64_db": evm64, "delta_db": evm_gpu - evm64,
                 "within_0p1_db": abs(evm_gpu - evm64) < 0.1},
     }
+    print(json.dumps(result), flush=True)
+
+
+def main_precode(args):
+    """Downlink precode of a 256x32 cell (mega_precode_syms): 3276 subcarriers
+    in 273 groups of 12 (one B x U precoder each), 13 OFDM symbols per step,
+    written at the FFT-shifted columns of 4096-point antenna rows (start 410).
+    A step precodes one slot; successive steps rotate four buffer sets (about
+    140 MB each) so that they are not served from the Infinity Cache. The
+    CPU baseline is the oracle's single-threaded C port on one symbol."""
+    import torch
+    import megastation_amd as m
+    B, U, N, sg, S, row, start = 256, 32, 3276, 12, 13, 4096, 410
+    G = N // sg
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(77)
+
+    def cplx(n):
+        return torch.complex(torch.randn(n, device=dev, generator=gen),
+                             torch.randn(n, device=dev, generator=gen))
+    NR = 4
+    P = [cplx(G * U * B) for _ in range(NR)]
+    X = [cplx(S * U * N) for _ in range(NR)]
+    O = [torch.zeros(S * B * row, dtype=torch.complex64, device=dev) for _ in range(NR)]
+    stream = torch.cuda.current_stream()
+    kw = dict(num_bs=B, num_ue=U, num_sc=N, sc_group=sg, bs_stride=row, bs_start=start,
+              num_syms=S, stream=stream)
+
+    def step(j):
+        m.precode(P[j % NR], X[j % NR], O[j % NR], **kw)
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    for j in range(args.warmup):
+        step(j)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for j in range(args.steps):
+        evs[j][0].record(stream)
+        step(j)
+        evs[j][1].record(stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / args.steps
+    k_ms = statistics.mean(a.elapsed_time(b) for a, b in evs)
+    symbols = U * G * sg * S
+    io = 8 * B * U * G + S * (8 * U * N + 8 * B * G * sg)
+    flops = 8 * B * U * G * sg * S
+    result = {
+        "metric": "precoded user symbols/s (downlink precode), 1 GPU",
+        "value": symbols / (ms * 1e-3), "unit": "symbols/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "d3_precode_256x32", "antennas": B, "users": U,
+                   "subcarriers": N, "sc_group": sg, "groups": G, "ofdm_symbols": S,
+                   "row": row, "bs_start": start, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": "precode_kernel",
+                     "achieved": io / (k_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": io / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic("d3_precode_256x32", "precode_kernel") or
+                                 {}).get("traffic_bytes"),
+                     "algorithmic_bytes": io, "avg_ms": k_ms,
+                     "tflops": flops / (k_ms * 1e-3) / 1e12},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        if not os.path.exists(oracle.LIB_PATH):
+            oracle.build()
+        p0 = P[0].cpu().numpy().reshape(G, U, B)
+        x0 = X[0][:U * N].cpu().numpy().reshape(U, N)
+        ts = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            oracle.precode(p0, x0, B, U, N, sg, row, start)
+            ts.append(time.perf_counter() - c0)
+        sec = statistics.median(ts)
+        result["cpu_baseline"] = {"value": U * G * sg / sec, "unit": "symbols/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": "1 of 13 symbols, median of 3, oracle C port "
+                                            "(single thread)"}
     print(json.dumps(result), flush=True)
 
 
@@ -583,7 +679,7 @@ def main_batch(args):
     io = sum(8 * c["B"] * c["N"] * c["S"] + 3 * 8 * c["U"] * c["B"] * ((c["N"] + c["sg"] - 1) // c["sg"])
              + 2 * c["U"] * c["Q"] * c["N"] * c["S"] for c in cells)
     # EVM of cell 0 (x_hat through the single-cell entry points of the same
-    # W) against the FP64 oracle's
+    # W) against FP64 zero-forcing's
     c0 = cells[0]
     U, B, N, sg, S, Q = (c0[x] for x in ("U", "B", "N", "sg", "S", "Q"))
     G = (N + sg - 1) // sg
@@ -679,20 +775,20 @@ def cpu_baseline(cfg, lay, csi, y, x, budget_s):
 
 
 def fp64_evm(cfg, csi, y, x0):
-    """EVM (dB) of the FP64 oracle's equalized symbols of slot 0 against the
-    transmitted ones: the reference every form's EVM is held to."""
-    import oracle
+    """EVM (dB) of FP64 zero-forcing of slot 0 against the transmitted
+    symbols: the reference every form's EVM is held to. Textbook ZF in numpy
+    complex128 (W_g = (A_g A_g^H)^-1 A_g by LAPACK solve, then x = W_g y per
+    subcarrier), independent of the library."""
     from megastation_amd import synth
-    if not os.path.exists(oracle.LIB_PATH):
-        oracle.build()
-    U, B, N, sg, S, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "Q"))
+    U, B, N, sg, S = (cfg[k] for k in ("U", "B", "N", "sg", "S"))
     G = (N + sg - 1) // sg
-    lay = synth.standalone_layout(N, Q)
-    w64, _, _ = oracle.zf(csi[0].cpu().numpy(), U, B, G, "f64")
-    y0 = y[0].cpu().numpy()
-    x64 = np.stack([oracle.equalize(w64, y0[s], U, B, N, sg, Q, lay.ue_stride,
-                                    lay.cb_size, lay.cb_stride, lay.zeros,
-                                    lay.valid_bits, "f64")[1] for s in range(S)])
+    A = csi[0].cpu().numpy().astype(np.complex128).reshape(G, B, U).transpose(0, 2, 1)
+    W = np.linalg.solve(A @ A.conj().transpose(0, 2, 1), A)  # [G][U][B]
+    y0 = y[0].cpu().numpy().astype(np.complex128).reshape(S, B, N)
+    yp = np.zeros((S, B, G * sg), np.complex128)
+    yp[:, :, :N] = y0
+    Y = yp.reshape(S, B, G, sg).transpose(0, 2, 1, 3)  # [S][G][B][sg]
+    x64 = (W[None] @ Y).transpose(0, 2, 1, 3).reshape(S, U, G * sg)[:, :, :N]
     return synth.evm_db(x64, x0)
 
 

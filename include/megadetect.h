@@ -159,6 +159,41 @@ typedef struct {
 
 int mega_equalize_slots(const mega_equalize_desc *desc, mega_stream_t stream);
 
+/* ---- Downlink precode ----------------------------------------------------- */
+
+/* Replaces the raw overload Precode::precode(problem_size_row = B,
+ * problem_size_col = U, batch_count = N, bPrecoded, bUeVec, bBsVec,
+ * bs_stride, precode_skip = sg, bs_start, stream)
+ * (src/operators/sgemv/precode.cu:20-57, precode.h:38-42): one OFDM symbol,
+ *   bsvec[b][(k + bs_start + bs_stride/2) % bs_stride] = sum_u P_g[b][u] ue[u][k]
+ * for g = k / sg, with P [G][U][B] (a column-major B x U precoder per group,
+ * e.g. mega_beamformer's output), ue [U][N] (user rows, subcarrier fastest)
+ * and bsvec [B][bs_stride]. As in the reference only the N / sg full groups
+ * are computed; the other entries of bsvec are left as they are. The Matrix
+ * overload (precode.cu:59-69) maps to this with B = precoded.dim(0),
+ * U = precoded.dim(1), N = ueVec.dim(0), bs_stride = bsVec.stride(1).
+ * num_ue > 64 is MEGA_ERR_UNSUPPORTED. */
+int mega_precode(uint64_t row, uint64_t col, uint64_t batch_count,
+                 const void *precode, const void *uevec, void *bsvec,
+                 uint64_t bs_stride, uint64_t precode_skip, uint64_t bs_start,
+                 mega_stream_t stream);
+
+/* Many OFDM symbols in one launch: symbol t reads uevec + t*ue_sym_stride
+ * and writes bsvec + t*bs_sym_stride (complex elements); each workgroup's
+ * precoder block is read once for its share of the symbols. Extends
+ * precode.cu:20-57, which handles one symbol per launch. */
+typedef struct {
+  uint32_t num_bs, num_ue, num_sc, sc_group, num_syms; /* B, U, N, sg, S */
+  uint64_t bs_stride, bs_start;
+  const void *precode;
+  const void *uevec;
+  uint64_t ue_sym_stride;
+  void *bsvec;
+  uint64_t bs_sym_stride;
+} mega_precode_desc;
+
+int mega_precode_syms(const mega_precode_desc *desc, mega_stream_t stream);
+
 /* ---- Fused detection ------------------------------------------------------
  * ZF weights and equalize + demod of every symbol of every slot in one call:
  * eq.csi holds the channel estimates A_g = H_g^H (not W). For U <= 16 and

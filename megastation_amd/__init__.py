@@ -89,6 +89,16 @@ def declared_symbols() -> list[str]:
                                  text, re.M)))
 
 
+class PrecodeDesc(ctypes.Structure):
+    """mega_precode_desc (include/megadetect.h)."""
+    _fields_ = [("num_bs", ctypes.c_uint32), ("num_ue", ctypes.c_uint32),
+                ("num_sc", ctypes.c_uint32), ("sc_group", ctypes.c_uint32),
+                ("num_syms", ctypes.c_uint32), ("bs_stride", ctypes.c_uint64),
+                ("bs_start", ctypes.c_uint64), ("precode", ctypes.c_void_p),
+                ("uevec", ctypes.c_void_p), ("ue_sym_stride", ctypes.c_uint64),
+                ("bsvec", ctypes.c_void_p), ("bs_sym_stride", ctypes.c_uint64)]
+
+
 def lib() -> ctypes.CDLL:
     """Load libmegadetect.so (raises if it has not been built)."""
     global _lib
@@ -117,6 +127,8 @@ def lib() -> ctypes.CDLL:
         "mega_detect_slots": ([ctypes.POINTER(DetectDesc), P], i32),
         "mega_detect_work_bytes": ([ctypes.POINTER(DetectDesc)], u64),
         "mega_demod_host": ([P, u64, u8, P], i32),
+        "mega_precode": ([u64, u64, u64, P, P, P, u64, u64, u64, P], i32),
+        "mega_precode_syms": ([ctypes.POINTER(PrecodeDesc), P], i32),
         "mega_batch_create": ([ctypes.POINTER(CellDesc), u32, ctypes.c_char_p,
                                ctypes.POINTER(P)], i32),
         "mega_batch_run": ([P, P], i32),
@@ -275,6 +287,17 @@ def equalize_slots(csi, y, llr, *, num_ue, num_bs, num_sc, sc_group, num_syms,
                      y_sym_stride, y_slot_stride, _ptr(llr).value, llr_sym_stride,
                      llr_slot_stride, _ptr(xhat).value)
     _check(lib().mega_equalize_slots(ctypes.byref(d), _stream(stream)))
+
+
+def precode(prec, ue, out, *, num_bs, num_ue, num_sc, sc_group, bs_stride, bs_start,
+            num_syms: int = 1, ue_sym_stride: int = 0, bs_sym_stride: int = 0, stream=None):
+    """Downlink precode (mega_precode_syms): prec [G][U][B] complex64, ue
+    [S][U][N], out [S][B][bs_stride]; symbol t at ue + t*ue_sym_stride and
+    out + t*bs_sym_stride (complex elements; default: dense)."""
+    d = PrecodeDesc(num_bs, num_ue, num_sc, sc_group, num_syms, bs_stride, bs_start,
+                    _ptr(prec).value, _ptr(ue).value, ue_sym_stride or num_ue * num_sc,
+                    _ptr(out).value, bs_sym_stride or num_bs * bs_stride)
+    _check(lib().mega_precode_syms(ctypes.byref(d), _stream(stream)))
 
 
 def detect_fused_shape(num_ue: int, num_bs: int) -> bool:

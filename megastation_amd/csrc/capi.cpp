@@ -307,6 +307,56 @@ int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
   return mega_equalize_slots(&d, stream);
 }
 
+int mega_precode_syms(const mega_precode_desc* d, mega_stream_t stream) {
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, "mega_precode: desc is NULL");
+  if (d->num_bs == 0 || d->num_ue == 0 || d->num_syms == 0 || d->sc_group == 0 ||
+      d->num_sc < d->sc_group)
+    return MEGA_OK;  // no full group: nothing to write (as the reference)
+  if (!d->precode || !d->uevec || !d->bsvec)
+    return fail(MEGA_ERR_INVALID_ARG, "mega_precode: NULL buffer");
+  if (d->bs_stride == 0) return fail(MEGA_ERR_INVALID_ARG, "mega_precode: bs_stride is 0");
+  if (d->num_ue > mega::detail::kMaxUsers)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_precode: num_ue > 64");
+  mega::detail::PrecodeArgs a;
+  a.prec = static_cast<const float2*>(d->precode);
+  a.ue = static_cast<const float2*>(d->uevec);
+  a.out = static_cast<float2*>(d->bsvec);
+  a.B = d->num_bs;
+  a.U = d->num_ue;
+  a.N = d->num_sc;
+  a.sg = d->sc_group;
+  a.S = d->num_syms;
+  a.row = d->bs_stride;
+  a.bs_start = d->bs_start;
+  a.ue_sym = d->ue_sym_stride;
+  a.out_sym = d->bs_sym_stride;
+  const char* why = nullptr;
+  hipError_t e = mega::detail::launch_precode(a, static_cast<hipStream_t>(stream), &why);
+  if (e != hipSuccess) return hip_fail(e, "mega_precode", why);
+  return MEGA_OK;
+}
+
+int mega_precode(uint64_t row, uint64_t col, uint64_t batch_count, const void* precode,
+                 const void* uevec, void* bsvec, uint64_t bs_stride, uint64_t precode_skip,
+                 uint64_t bs_start, mega_stream_t stream) {
+  if (row > 0xffffffffull || col > 0xffffffffull || batch_count > 0xffffffffull ||
+      precode_skip > 0xffffffffull)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_precode: size exceeds 32 bits");
+  mega_precode_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.num_bs = (uint32_t)row;
+  d.num_ue = (uint32_t)col;
+  d.num_sc = (uint32_t)batch_count;
+  d.sc_group = (uint32_t)precode_skip;
+  d.num_syms = 1;
+  d.precode = precode;
+  d.uevec = uevec;
+  d.bsvec = bsvec;
+  d.bs_stride = bs_stride;
+  d.bs_start = bs_start;
+  return mega_precode_syms(&d, stream);
+}
+
 static int hip_status(hipError_t e, const char* who, const char* why = nullptr) {
   return e == hipSuccess ? MEGA_OK : hip_fail(e, who, why);
 }

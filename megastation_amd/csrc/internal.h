@@ -72,6 +72,18 @@ hipError_t launch_learned_finish(uint32_t U, uint32_t B, uint32_t batch, const f
                                  const float2* Gm, float2* X, const int32_t* flag, float2* w,
                                  int32_t* info, hipStream_t st);
 
+// Downlink precode of S symbols (precode.hip): prec [G][U][B], ue [U][N] per
+// symbol (ue_sym apart), out [B][row] per symbol (out_sym apart); strides in
+// complex elements.
+struct PrecodeArgs {
+  const float2* prec;
+  const float2* ue;
+  float2* out;
+  uint32_t B, U, N, sg, S;
+  uint64_t row, bs_start, ue_sym, out_sym;
+};
+hipError_t launch_precode(const PrecodeArgs& a, hipStream_t stream, const char** why);
+
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,
                                hipStream_t stream);

@@ -64,6 +64,8 @@ def lib() -> ctypes.CDLL:
         L.orc_zf_f32.restype = i32
         L.orc_scale_absmax.argtypes = [u32, u32, u32, P, P]
         L.orc_scale_absmax.restype = None
+        L.orc_precode.argtypes = [u32, u32, u32, u32, P, P, P, ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_precode.restype = None
         L.orc_detect_cpu.argtypes = [u32, u32, u32, u32, u32, u32, i32, P, P, P,
                                      u64, u64, u64, u64, u64, i32]
         L.orc_detect_cpu.restype = i32
@@ -150,6 +152,19 @@ def scale_absmax(w: np.ndarray, U: int, B: int, batch: int) -> np.ndarray:
     out = np.zeros_like(a)
     lib().orc_scale_absmax(U, B, batch, _p(a), _p(out))
     return out
+
+
+def precode(prec: np.ndarray, ue: np.ndarray, B: int, U: int, N: int, sg: int,
+            bs_stride: int, bs_start: int, out: np.ndarray | None = None) -> np.ndarray:
+    """Downlink precode of one symbol (orc_precode): prec [G][U][B], ue [U][N],
+    returns out [B][bs_stride] complex64 (a copy of `out` if given, else
+    zeros, with the written entries replaced)."""
+    p = _c64(prec).reshape(-1)
+    x = _c64(ue).reshape(-1)
+    o = (np.zeros(B * bs_stride, np.complex64) if out is None
+         else _c64(out).reshape(-1).copy())
+    lib().orc_precode(B, U, N, sg, _p(p), _p(x), _p(o), bs_stride, bs_start)
+    return o.reshape(B, bs_stride)
 
 
 def detect_cpu(csi: np.ndarray, y: np.ndarray, slots: int, S: int, U: int,

@@ -22,6 +22,10 @@
  *                                    (1e-5), numpy.linalg.pinv, W A^H = I,
  *                                    ZF(ZF(A)) = A
  *   orc_scale_absmax               - reference fixture test/beamform/precoder.data
+ *   orc_precode                    - reference fixture test/sgemv/precoded.data
+ *                                    (1e-5), numpy complex128 products with the
+ *                                    shifted placement, identity precoder,
+ *                                    untouched partial group and guard columns
  *   orc_detect_cpu                 - equal, bit for bit, to orc_zf_f32 followed by
  *                                    orc_equalize_f32
  */
@@ -431,6 +435,39 @@ void orc_scale_absmax(uint32_t U, uint32_t B, uint32_t batch, const float *in,
       /* input * Complex(1/max, 0) */
       o[2 * t] = re * s - im * 0.f;
       o[2 * t + 1] = re * 0.f + im * s;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* downlink precode (precode_kernel.cuh:418-452, the naive specification;      */
+/* layouts of the tiled kernel, :180-187 and :292-300; grid precode.cu:25-28) */
+/* ------------------------------------------------------------------------ */
+
+/* out[b][(k + bs_start + bs_stride/2) % bs_stride] = sum_u P_g[b][u] ue[u][k]
+ * for g = k / sg and k < (N / sg) * sg: the reference launches N / sg
+ * groups (floor), so the subcarriers of a partial last group and the columns
+ * no subcarrier maps to are left as they are. P is [G][U][B] (column-major
+ * B x U per group), ue [U][N], out [B][bs_stride]; complex64 interleaved.
+ * Products and sums in the order of mega::Complex (mega_complex.h:35-49),
+ * users in increasing order. */
+void orc_precode(uint32_t B, uint32_t U, uint32_t N, uint32_t sg, const float *prec,
+                 const float *ue, float *out, uint64_t bs_stride, uint64_t bs_start) {
+  const uint32_t G = N / sg;
+  for (uint32_t k = 0; k < G * sg; ++k) {
+    const float *P = prec + 2 * (size_t)(k / sg) * U * B;
+    const uint64_t col = (k + bs_start + bs_stride / 2) % bs_stride;
+    for (uint32_t b = 0; b < B; ++b) {
+      float re = 0.f, im = 0.f;
+      for (uint32_t u = 0; u < U; ++u) {
+        const float pr = P[2 * ((size_t)u * B + b)], pi = P[2 * ((size_t)u * B + b) + 1];
+        const float xr = ue[2 * ((size_t)u * N + k)], xi = ue[2 * ((size_t)u * N + k) + 1];
+        const float tr = pr * xr - pi * xi, ti = pr * xi + pi * xr;
+        re = re + tr;
+        im = im + ti;
+      }
+      out[2 * (b * bs_stride + col)] = re;
+      out[2 * (b * bs_stride + col) + 1] = im;
     }
   }
 }

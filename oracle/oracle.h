@@ -75,6 +75,12 @@ int orc_zf_f32(uint32_t U, uint32_t B, uint32_t batch, const float *in,
 void orc_scale_absmax(uint32_t U, uint32_t B, uint32_t batch, const float *in,
                       float *out);
 
+/* Downlink precode of one OFDM symbol (precode.cu:20-57): out[b][(k +
+ * bs_start + bs_stride/2) % bs_stride] = sum_u P_{k/sg}[b][u] ue[u][k] for
+ * the N/sg full groups; other output entries untouched. */
+void orc_precode(uint32_t B, uint32_t U, uint32_t N, uint32_t sg, const float *prec,
+                 const float *ue, float *out, uint64_t bs_stride, uint64_t bs_start);
+
 /* ---- CPU baseline: the same operator math fused per subcarrier group,
  * OpenMP over groups (FP32 Gram -> Cholesky -> W, then S symbols of
  * equalize + demod). Slots are laid out back to back:

@@ -3,10 +3,11 @@
 //
 //   test_dropin host            descriptor tests + demod known answer (no GPU)
 //   test_dropin gpu <dir>       all of the above with device buffers, plus the
-//                               equalize and beamform fixtures read from <dir>
+//                               equalize, precode and beamform fixtures read
+//                               from <dir>
 //
 // Mirrors test/matrix/test_matrix.cc, test/matrix/test_cuphy_tensor.cc,
-// test/modulation/test_modulation.cc:11-57, test/sgemv/test_sgemv.cc:126-159
+// test/modulation/test_modulation.cc:11-57, test/sgemv/test_sgemv.cc:126-196
 // and test/beamform/test_beamform.cc:67-95.
 #include <hip/hip_runtime.h>
 
@@ -24,6 +25,7 @@
 #include "mega/matrix.h"
 #include "mega/mega_complex.h"
 #include "mega/padded_tensor.h"
+#include "mega/precode.h"
 
 static int g_fail = 0, g_checks = 0;
 #define EXPECT(c)                                                        \
@@ -199,6 +201,29 @@ static void equalize_fixture(const std::string& dir) {
   EXPECT(zero);
 }
 
+static void precode_fixture(const std::string& dir) {
+  std::printf("Precode fixture\n");
+  const uint32_t ue = 4, bs = 8, ofdm = 16, ofdmE = 32;
+  auto prec = read_file<mega::Complex>(dir + "/precode_prec.bin");
+  auto uev = read_file<mega::Complex>(dir + "/precode_uevec_ku.bin");
+  auto ref = read_file<float>(dir + "/precode_out.bin");
+  Matrix d_prec(sizeof(mega::Complex), bs, ue, Matrix::kDevice);
+  Matrix d_ue(sizeof(mega::Complex), ofdm, ue, Matrix::kDevice);
+  Matrix d_out(sizeof(mega::Complex), ofdmE, bs, Matrix::kDevice);
+  EXPECT(prec.size() * 8 == d_prec.szBytes() && uev.size() * 8 == d_ue.szBytes());
+  EXPECT(ref.size() * 4 == d_out.szBytes());
+  std::vector<mega::Complex> t(ue * ofdm);
+  for (uint32_t i = 0; i < ofdm; ++i)
+    for (uint32_t j = 0; j < ue; ++j) t[j * ofdm + i] = uev[i * ue + j];
+  EXPECT(hipMemcpy(d_prec.ptr(), prec.data(), d_prec.szBytes(), hipMemcpyHostToDevice) == hipSuccess);
+  EXPECT(hipMemcpy(d_ue.ptr(), t.data(), d_ue.szBytes(), hipMemcpyHostToDevice) == hipSuccess);
+  EXPECT(hipMemset(d_out.ptr(), 0, d_out.szBytes()) == hipSuccess);
+  mega::Precode::precode(d_prec, d_ue, d_out, ofdm, 8);
+  std::vector<float> out(2 * bs * ofdmE);
+  EXPECT(hipMemcpy(out.data(), d_out.ptr(), d_out.szBytes(), hipMemcpyDeviceToHost) == hipSuccess);
+  for (uint32_t i = 0; i < 2 * bs * ofdmE; ++i) EXPECT(std::fabs(out[i] - ref[i]) < 1e-5f);
+}
+
 static void beamform_fixture(const std::string& dir) {
   std::printf("Beamform fixture\n");
   const uint32_t ue = 4, bs = 8;
@@ -252,6 +277,7 @@ int main(int argc, char** argv) {
     demod_known_answer();
     if (mode == "gpu") {
       equalize_fixture(argv[2]);
+      precode_fixture(argv[2]);
       beamform_fixture(argv[2]);
     }
   } catch (const std::exception& e) {

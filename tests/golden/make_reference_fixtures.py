@@ -11,6 +11,12 @@ Sources (binary fixtures next to the reference tests):
   test/beamform/csi_in.data   U=4 x B=8 CSI (test/beamform/test_beamform.cc:19-33)
   test/beamform/csi_out.data  expected ZF weights, same layout (1e-5, :91)
   test/beamform/precoder.data expected normalised precoder, B x U (1e-5, :92)
+  test/sgemv/prec.data      precoder, bs=8 x ue=4 column-major complex64, one
+                            group of 16 subcarriers (test_sgemv.cc:161-196)
+  test/sgemv/ueVec.data     UE symbols as [ofdm=16][ue=4] complex64; the test
+                            transposes them to [ue][ofdm] before upload
+  test/sgemv/precoded.data  expected output, [bs=8][ofdmE=32] complex64, for
+                            precode_skip = 16 and bs_start = 8 (1e-5)
 
 The arrays are stored unchanged (raw little-endian values); this script only
 re-packages them so the tests run where the reference tree is absent.
@@ -38,11 +44,17 @@ def main():
         "beam_csi_in": rd("test/beamform/csi_in.data", np.complex64),
         "beam_csi_out": rd("test/beamform/csi_out.data", np.complex64),
         "beam_precoder": rd("test/beamform/precoder.data", np.complex64),
+        "precode_prec": rd("test/sgemv/prec.data", np.complex64),
+        "precode_uevec_ku": rd("test/sgemv/ueVec.data", np.complex64),
+        "precode_out": rd("test/sgemv/precoded.data", np.complex64),
     }
     assert arrays["sgemv_csi"].size == 4 * 8
     assert arrays["sgemv_bsvec_kb"].size == 16 * 8
     assert arrays["sgemv_equalize_f16bits"].size == 4 * 32
     assert arrays["beam_csi_in"].size == 4 * 8
+    assert arrays["precode_prec"].size == 8 * 4
+    assert arrays["precode_uevec_ku"].size == 16 * 4
+    assert arrays["precode_out"].size == 8 * 32
     np.savez(OUT, **arrays)
     print("wrote", OUT)
 

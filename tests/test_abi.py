@@ -88,6 +88,18 @@ def test_bad_pipeline_arguments():
     assert L.mega_pipeline_graph_destroy(None) == m.MEGA_OK
 
 
+def test_bad_precode_arguments():
+    L = m.lib()
+    assert L.mega_precode(8, 65, 16, 1, 1, 1, 32, 16, 0, None) == m.MEGA_ERR_UNSUPPORTED
+    assert L.mega_precode(8, 4, 16, None, 1, 1, 32, 16, 0, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_precode(8, 4, 16, 1, 1, 1, 0, 16, 0, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_precode(1 << 33, 4, 16, 1, 1, 1, 32, 16, 0, None) == m.MEGA_ERR_UNSUPPORTED
+    # no full group (N < sg): nothing to write, no buffer needed
+    assert L.mega_precode(8, 4, 8, None, None, None, 32, 16, 0, None) == m.MEGA_OK
+    assert L.mega_precode_syms(None, None) == m.MEGA_ERR_INVALID_ARG
+    assert ctypes.sizeof(m.PrecodeDesc) == 80
+
+
 def test_equalize_desc_layout_matches_header():
     # offsets of the C struct on LP64 (natural alignment)
     D = m.EqualizeDesc

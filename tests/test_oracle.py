@@ -220,6 +220,53 @@ def test_precoder_reference_fixture(orc, golden):
         assert np.abs(p[32 * g: 32 * g + 32] - golden["beam_precoder"]).max() < 1e-6
 
 
+def test_precode_reference_fixture(orc, golden):
+    """test/sgemv/test_sgemv.cc:161-196: bs=8, ue=4, 16 subcarriers in one
+    group (precode_skip 16), output rows of 32, bs_start 8; the UE symbols
+    are transposed from [ofdm][ue] to [ue][ofdm] before upload; 1e-5."""
+    prec = golden["precode_prec"]
+    ue = golden["precode_uevec_ku"].reshape(16, 4).T.copy()
+    out = orc.precode(prec, ue, 8, 4, 16, 16, 32, 8)
+    ref = golden["precode_out"].reshape(8, 32)
+    assert np.abs(out - ref).max() < 1e-5
+    # the placement: subcarrier k at column (k + 8 + 16) % 32, the rest zero
+    assert not out[:, 8:24].any() and not ref[:, 8:24].any()
+
+
+def test_precode_brute_force_and_untouched_columns(orc):
+    """Against numpy complex128 matrix products per group: a partial last
+    group (N % sg != 0) and the guard columns keep what the buffer held (the
+    reference launches N / sg groups), every other column holds P_g ue_k."""
+    rng = np.random.default_rng(11)
+    B, U, N, sg, row, start = 24, 5, 100, 12, 128, 17
+    G = (N + sg - 1) // sg
+    P = (rng.standard_normal((G, U, B)) + 1j * rng.standard_normal((G, U, B))).astype(np.complex64)
+    ue = (rng.standard_normal((U, N)) + 1j * rng.standard_normal((U, N))).astype(np.complex64)
+    fill = np.full((B, row), 7 - 3j, np.complex64)
+    out = orc.precode(P, ue, B, U, N, sg, row, start, out=fill)
+    written = set()
+    for k in range(N // sg * sg):
+        col = (k + start + row // 2) % row
+        written.add(col)
+        exp = P[k // sg].astype(np.complex128).T @ ue[:, k].astype(np.complex128)
+        np.testing.assert_allclose(out[:, col], exp, rtol=1e-5, atol=1e-5)
+    for col in set(range(row)) - written:
+        assert (out[:, col] == 7 - 3j).all(), col
+    assert len(written) == 96
+
+
+def test_precode_identity_is_exact(orc):
+    """P = I places the UE symbols unchanged (1*x + 0*y is exact)."""
+    rng = np.random.default_rng(3)
+    U = B = 8
+    N, sg, row, start = 48, 16, 64, 5
+    P = np.tile(np.eye(B, dtype=np.complex64)[None], (N // sg, 1, 1))
+    ue = (rng.standard_normal((U, N)) + 1j * rng.standard_normal((U, N))).astype(np.complex64)
+    out = orc.precode(P, ue, B, U, N, sg, row, start)
+    for k in range(N):
+        assert np.array_equal(out[:, (k + start + row // 2) % row], ue[:, k])
+
+
 def test_zf_matches_pinv_and_invariants(orc):
     rng = np.random.default_rng(11)
     U, B, G = 6, 20, 7
