@@ -238,16 +238,33 @@ def main():
     # steps with no host work between them.
     from megastation_amd.pipeline import PipelineGraph
 
-    def make_graph(mixed):
+    # Two slots per push: pairs of consecutive slots in contiguous buffers
+    # (the four rotating slots as two pairs); the ZF of a pair is one launch
+    # over both slots' groups, its equalize one launch.
+    pair_r = []
+
+    def pairs():
+        if not pair_r:
+            for j in range(0, NR, 2):
+                pair_r.append((torch.cat([csi_r[j], csi_r[j + 1]]),
+                               torch.cat([y_r[j], y_r[j + 1]]),
+                               torch.cat([llr_r[j], llr_r[j + 1]])))
+        return pair_r
+
+    def make_graph(mixed, per_push=1):
+        if per_push == 2:
+            pr = pairs()
+            pushes = [pr[i % len(pr)] for i in range(max(1, args.steps // 2))]
+        else:
+            pushes = [(csi_r[i % NR], y_r[i % NR], llr_r[i % NR]) for i in range(args.steps)]
         return PipelineGraph(
-            [(csi_r[i % NR], y_r[i % NR], llr_r[i % NR]) for i in range(args.steps)],
-            num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, mod_order=Q,
+            pushes, num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, mod_order=Q,
             ue_stride=lay.ue_stride, cb_size=lay.cb_size, cb_stride=lay.cb_stride,
             zeros=lay.zeros, valid_bits=lay.valid_bits,
-            form=m.FORM_MIXED if mixed else m.FORM_FP32)
+            form=m.FORM_MIXED if mixed else m.FORM_FP32, slots=per_push), per_push * len(pushes)
 
-    def timed_graph(gr):
-        """W warm-up launches, then one timed launch of exactly K steps."""
+    def timed_graph(gr, steps):
+        """W warm-up launches, then one timed launch of `steps` steps."""
         for _ in range(max(1, args.warmup)):
             gr.launch(stream)
         torch.cuda.synchronize()
@@ -257,7 +274,7 @@ def main():
         gr.launch(stream)
         t1.record(stream)
         torch.cuda.synchronize()
-        return t0.elapsed_time(t1) / args.steps, None, None
+        return t0.elapsed_time(t1) / steps, None, None
 
     if args.copy_calibration:
         src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
@@ -314,9 +331,10 @@ def main():
         for name, mixed in (("plain_pipe", False), ("mixed_pipe", True)):
             forms[name] = timed(lambda evs=None, mx=mixed: step_pipe(mx), pipe=get_pipe(mixed))
             print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
-        for name, mixed in (("plain_graph", False), ("mixed_graph", True)):
-            gr = make_graph(mixed)
-            forms[name] = timed_graph(gr)
+        for name, mixed, per in (("plain_graph", False, 1), ("mixed_graph", True, 1),
+                                 ("plain_graph2", False, 2), ("mixed_graph2", True, 2)):
+            gr, steps = make_graph(mixed, per)
+            forms[name] = timed_graph(gr, steps)
             gr.close()
             print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
 
@@ -333,7 +351,7 @@ def main():
     x0 = x[0].cpu().numpy()
     evm = {}
     for name in forms:
-        if forms[name] is None or name.endswith(("_pipe", "_graph")):
+        if forms[name] is None or name.endswith(("_pipe", "_graph", "_graph2")):
             continue
         xh.zero_()
         if name == "fused":
@@ -348,7 +366,7 @@ def main():
         torch.cuda.synchronize()
         evm[name] = synth.evm_db(xh.cpu().numpy().reshape(S, U, N), x0)
     for name in forms:
-        if name.endswith(("_pipe", "_graph")):  # same kernels and inputs as the sequential form
+        if name.endswith(("_pipe", "_graph", "_graph2")):  # same kernels as the sequential form
             evm[name] = evm[name.rsplit("_", 1)[0]]
     evm64 = fp64_evm(cfg, csi, y, x0)
     ok = {k: abs(v - evm64) < 0.1 for k, v in evm.items()}
@@ -360,7 +378,7 @@ def main():
     if head_form == "fused":
         dom_kernel, dom_ms, dom_prefix = ("detect_fused (ZF + equalize + demod)",
                                           forms["fused"][1], "detect_fused")
-    elif head_form in ("plain", "plain_pipe", "plain_graph"):
+    elif head_form.startswith("plain"):
         dom_kernel, dom_ms, dom_prefix = "equalize_demod", forms["plain"][2], "equalize_"
     else:
         dom_kernel, dom_ms, dom_prefix = ("equalize_demod (split BF16)", forms["mixed"][2],
@@ -385,6 +403,10 @@ def main():
         elif k.endswith("_graph"):
             e["schedule"] = (f"the {k[:-6]}_pipe schedule of {args.steps} consecutive steps "
                              "captured into one graph, one launch")
+        elif k.endswith("_graph2"):
+            e["schedule"] = (f"two consecutive slots per push (ZF of both slots' groups in one "
+                             f"launch, equalize of both in one), the pipelined schedule of "
+                             f"{2 * max(1, args.steps // 2)} steps captured into one graph")
         elif k != "fused":
             e.update({"zf_ms": t1, "equalize_ms": t2,
                       "zf_tflops": zf_flops / (t1 * 1e-3) / 1e12,

@@ -275,9 +275,14 @@ int mega_batch_destroy(mega_batch_t batch);
  * bit for bit. No single reference counterpart: it schedules the
  * Beamform::beamform_uplink + Equalize::equalize calls of consecutive frames
  * (process.cc:172-185, gpu_engine.cc:289-343). `shape` gives the cell:
- * num_ue .. valid_bits, form (MEGA_FORM_FP32 or MEGA_FORM_MIXED) and the y /
- * llr symbol strides; its buffers, num_slots and slot strides are ignored.
- * A slot's csi, y, llr and xhat (optional) must stay valid until its
+ * num_ue .. valid_bits, form (MEGA_FORM_FP32 or MEGA_FORM_MIXED), the y /
+ * llr symbol strides and num_slots, the slots per push (>= 1): a push takes
+ * that many consecutive slots (csi dense, [slots][G][B][U]; y and llr at
+ * y_slot_stride / llr_slot_stride, 0 = dense; xhat dense) and runs their ZF
+ * as one launch and their equalize as one launch (more groups per launch:
+ * at 256x32 two slots per push cost 22% less per slot than one). Its buffer
+ * pointers and csi_slot_stride are ignored. A push's csi, y, llr and xhat
+ * (optional) must stay valid until its
  * equalize has run on `stream` (issued by the next push or by flush). One
  * thread per handle, on the device current at create; destroy waits for the
  * pipeline's work. Group failures are not reported (use mega_zf for that). */
@@ -294,8 +299,9 @@ mega_stream_t mega_pipeline_zf_stream(mega_pipeline_t pipe);
 int mega_pipeline_destroy(mega_pipeline_t pipe);
 
 /* n consecutive slots of one cell with their buffers bound (csi[i], y[i],
- * llr[i], and xhat[i] if xhat is not NULL), scheduled as n pushes and a
- * flush of the slot pipeline above and captured once into a graph (weights
+ * llr[i], and xhat[i] if xhat is not NULL; each holds shape->num_slots
+ * slots), scheduled as n pushes and a flush of the slot pipeline above and
+ * captured once into a graph (weights
  * in three buffers the handle owns). mega_pipeline_graph_launch runs all n
  * slots with one launch on `stream`, after the work issued there before it;
  * results equal those of the pipeline (and of the sequential calls) bit for

@@ -98,12 +98,15 @@ int mega_pipeline_create(const mega_equalize_desc* shape, uint32_t depth, mega_s
   if (r != MEGA_OK) return r;
   if (empty)
     return mega::detail::set_error(MEGA_ERR_INVALID_ARG, "mega_pipeline_create: empty cell");
-  a.slots = 1;
+  // a.slots = shape->num_slots: slots per push (their ZF is one launch over
+  // slots * G groups, their equalize one launch)
   const uint64_t Bp = mega::detail::split_row(a.B);
   a.csi_slot = shape->form == MEGA_FORM_MIXED ? (uint64_t)a.G * a.U * Bp
                                               : (uint64_t)a.G * a.B * a.U;
-  a.y_slot = (uint64_t)a.S * a.y_sym;
-  a.llr_slot = (uint64_t)a.S * a.llr_sym;
+  if (a.y_slot == 0) a.y_slot = (uint64_t)a.S * a.y_sym;
+  if (a.llr_slot == 0) a.llr_slot = (uint64_t)a.S * a.llr_sym;
+  if ((uint64_t)a.slots * a.G > 0xffffffffull)
+    return mega::detail::set_error(MEGA_ERR_UNSUPPORTED, "mega_pipeline_create: too many groups");
   mega_pipeline* p = new mega_pipeline();
   p->a = a;
   p->form = shape->form;
@@ -114,7 +117,7 @@ int mega_pipeline_create(const mega_equalize_desc* shape, uint32_t depth, mega_s
   p->eq_ev.assign(depth, nullptr);
   p->eq_rec.assign(depth, false);
   hipError_t e = hipStreamCreateWithFlags(&p->zs, hipStreamNonBlocking);
-  const size_t wbytes = a.csi_slot * 8;  // complex64 or a split word pair
+  const size_t wbytes = a.csi_slot * 8 * a.slots;  // complex64 or a split word pair
   for (uint32_t k = 0; k < depth && e == hipSuccess; ++k) {
     e = hipMalloc(&p->w[k], wbytes);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->zf_ev[k], hipEventDisableTiming);
@@ -141,10 +144,10 @@ int mega_pipeline_push(mega_pipeline_t p, const void* csi, const void* y, void* 
   const char* why = nullptr;
   const float2* in = static_cast<const float2*>(csi);
   e = p->form == MEGA_FORM_MIXED
-          ? mega::detail::launch_zf_mixed(p->a.U, p->a.B, p->a.G, in, nullptr, p->w[k], nullptr,
-                                          nullptr, p->zs, &why)
-          : mega::detail::launch_zf(p->a.U, p->a.B, p->a.G, in, static_cast<float2*>(p->w[k]),
-                                    nullptr, p->zs, &why);
+          ? mega::detail::launch_zf_mixed(p->a.U, p->a.B, p->a.slots * p->a.G, in, nullptr,
+                                          p->w[k], nullptr, nullptr, p->zs, &why)
+          : mega::detail::launch_zf(p->a.U, p->a.B, p->a.slots * p->a.G, in,
+                                    static_cast<float2*>(p->w[k]), nullptr, p->zs, &why);
   if (e != hipSuccess) return hip_err(e, "mega_pipeline_push", why);
   e = hipEventRecord(p->zf_ev[k], p->zs);
   if (e != hipSuccess) return hip_err(e, "mega_pipeline_push");

@@ -17,7 +17,7 @@ captures their pipelined schedule into one graph: one launch runs all n
 slots, with no host work per slot.
 
 Both forms: FORM_FP32 (mega_zf + mega_equalize_slots) and FORM_MIXED
-(mega_zf_split + the split-BF16 equalize). One slot per push; csi is
+(mega_zf_split + the split-BF16 equalize). One slot per push by default (`slots=n`: n consecutive dense slots per push, their ZF and equalize one launch each); csi is
 [G][B][U] complex64 (A = H^H), read on the pipeline's ZF stream `zs` (produce
 it there, or make `zs` wait for its producer); y [S][B][N] complex64 and llr,
 the slot's LLR rows ([S][U][ue_stride] halves), on the caller's stream.
@@ -31,10 +31,12 @@ from . import FORM_FP32, FORM_MIXED, EqualizeDesc, _check, _ptr, _stream, lib
 
 
 def _shape_desc(num_ue, num_bs, num_sc, sc_group, num_syms, mod_order, ue_stride, cb_size,
-                cb_stride, zeros, valid_bits, form):
+                cb_stride, zeros, valid_bits, form, slots=1):
+    if slots < 1:
+        raise ValueError("slots per push must be at least 1")
     d = EqualizeDesc()
     d.num_ue, d.num_bs, d.num_sc, d.sc_group = num_ue, num_bs, num_sc, sc_group
-    d.num_syms, d.num_slots, d.mod_order, d.form = num_syms, 1, mod_order, form
+    d.num_syms, d.num_slots, d.mod_order, d.form = num_syms, slots, mod_order, form
     d.ue_stride, d.cb_size, d.cb_stride = ue_stride, cb_size, cb_stride
     d.zeros, d.valid_bits = zeros, valid_bits
     d.y_sym_stride = num_bs * num_sc
@@ -52,12 +54,12 @@ def _check_form(form, depth=3):
 class SlotPipeline:
     def __init__(self, *, num_ue, num_bs, num_sc, sc_group, num_syms, mod_order, ue_stride,
                  cb_size, cb_stride, zeros, valid_bits, form: int = FORM_MIXED,
-                 stream=None, depth: int = 3):
+                 stream=None, depth: int = 3, slots: int = 1):
         import torch
         _check_form(form, depth)
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         d = _shape_desc(num_ue, num_bs, num_sc, sc_group, num_syms, mod_order, ue_stride,
-                        cb_size, cb_stride, zeros, valid_bits, form)
+                        cb_size, cb_stride, zeros, valid_bits, form, slots)
         self._h = ctypes.c_void_p()
         _check(lib().mega_pipeline_create(ctypes.byref(d), depth, _stream(self.stream),
                                           ctypes.byref(self._h)))
@@ -93,26 +95,28 @@ class SlotPipeline:
 
 
 class PipelineGraph:
-    """n consecutive slots of one cell, buffers bound, pipelined as SlotPipeline
-    would and captured into one graph; launch() runs them all. slots: a list
-    of (csi, y, llr) or (csi, y, llr, xhat) tuples of device tensors (xhat
-    given for all slots or none), which stay referenced by the handle."""
+    """n consecutive pushes of one cell, buffers bound, pipelined as
+    SlotPipeline would and captured into one graph; launch() runs them all.
+    pushes: a list of (csi, y, llr) or (csi, y, llr, xhat) tuples of device
+    tensors (xhat given for all or none), each holding `slots` dense slots;
+    they stay referenced by the handle."""
 
-    def __init__(self, slots, *, num_ue, num_bs, num_sc, sc_group, num_syms, mod_order,
-                 ue_stride, cb_size, cb_stride, zeros, valid_bits, form: int = FORM_MIXED):
+    def __init__(self, pushes, *, num_ue, num_bs, num_sc, sc_group, num_syms, mod_order,
+                 ue_stride, cb_size, cb_stride, zeros, valid_bits, form: int = FORM_MIXED,
+                 slots: int = 1):
         _check_form(form)
-        if not slots:
+        if not pushes:
             raise ValueError("no slots")
-        with_x = len(slots[0]) > 3 and slots[0][3] is not None
-        n = len(slots)
+        with_x = len(pushes[0]) > 3 and pushes[0][3] is not None
+        n = len(pushes)
         arr = ctypes.c_void_p * n
-        self._keep = [tuple(s_) for s_ in slots]
-        csi = arr(*(_ptr(s_[0]) for s_ in slots))
-        y = arr(*(_ptr(s_[1]) for s_ in slots))
-        llr = arr(*(_ptr(s_[2]) for s_ in slots))
-        xh = arr(*(_ptr(s_[3]) for s_ in slots)) if with_x else None
+        self._keep = [tuple(s_) for s_ in pushes]
+        csi = arr(*(_ptr(s_[0]) for s_ in pushes))
+        y = arr(*(_ptr(s_[1]) for s_ in pushes))
+        llr = arr(*(_ptr(s_[2]) for s_ in pushes))
+        xh = arr(*(_ptr(s_[3]) for s_ in pushes)) if with_x else None
         d = _shape_desc(num_ue, num_bs, num_sc, sc_group, num_syms, mod_order, ue_stride,
-                        cb_size, cb_stride, zeros, valid_bits, form)
+                        cb_size, cb_stride, zeros, valid_bits, form, slots)
         self._h = ctypes.c_void_p()
         _check(lib().mega_pipeline_graph_create(ctypes.byref(d), n, csi, y, llr, xh,
                                                 ctypes.byref(self._h)))

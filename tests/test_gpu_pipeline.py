@@ -140,6 +140,61 @@ def test_pipeline_graph_equals_sequential(form):
     gr.close()
 
 
+@pytest.mark.parametrize("form", [m.FORM_FP32, m.FORM_MIXED])
+def test_pipeline_two_slots_per_push(form):
+    """slots=2: a push takes two consecutive slots (one ZF launch over both
+    slots' groups, one equalize launch); three pushes eager and as a graph,
+    bit-equal to zf + equalize_slots called on the same two slots."""
+    print(f"case: pipeline two slots per push form={form}", flush=True)
+    t = torch()
+    U, B, N, sg, S, Q, n_push = 32, 256, 600, 12, 2, 6, 3
+    G = (N + sg - 1) // sg
+    lay = synth.standalone_layout(N, Q)
+    rng = np.random.Generator(np.random.Philox(17 + form))
+    pairs = [[synth.make_slot(rng, U, B, N, sg, S, Q, 25.0, "iid") for _ in range(2)]
+             for _ in range(n_push)]
+    csi = [dev_c64(np.concatenate([s_.csi.reshape(-1) for s_ in p_])) for p_ in pairs]
+    y = [dev_c64(np.concatenate([s_.y.reshape(-1) for s_ in p_])) for p_ in pairs]
+    n_llr = 2 * S * U * lay.ue_stride
+    kw = dict(num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, mod_order=Q,
+              ue_stride=lay.ue_stride, cb_size=lay.cb_size, cb_stride=lay.cb_stride,
+              zeros=lay.zeros, valid_bits=lay.valid_bits)
+    Bp = (B + 15) // 16 * 16
+    seq = []
+    for i in range(n_push):
+        llr = llr_buffer(n_llr)
+        if form == m.FORM_MIXED:
+            w = t.empty(2 * G * U * Bp, dtype=t.int64, device="cuda:0")
+            m.zf_split(csi[i], w, B, U, 2 * G)
+        else:
+            w = t.empty(2 * G * B * U, dtype=t.complex64, device="cuda:0")
+            m.zf(csi[i], w, B, U, 2 * G)
+        m.equalize_slots(w, y[i], llr, form=form, num_slots=2, csi_slot_stride=w.numel() // 2,
+                         y_sym_stride=B * N, y_slot_stride=S * B * N,
+                         llr_sym_stride=U * lay.ue_stride, llr_slot_stride=S * U * lay.ue_stride,
+                         **kw)
+        seq.append(host_u16(llr))
+    t.cuda.synchronize()
+    p = SlotPipeline(form=form, slots=2, **kw)
+    p.zs.wait_stream(t.cuda.current_stream())
+    outs = [llr_buffer(n_llr) for _ in range(n_push)]
+    for i in range(n_push):
+        p.push(csi[i], y[i], outs[i])
+    p.flush()
+    t.cuda.synchronize()
+    p.close()
+    for i in range(n_push):
+        assert np.array_equal(host_u16(outs[i]), seq[i]), i
+    outs_g = [llr_buffer(n_llr) for _ in range(n_push)]
+    gr = PipelineGraph([(csi[i], y[i], outs_g[i]) for i in range(n_push)], form=form, slots=2,
+                       **kw)
+    gr.launch()
+    t.cuda.synchronize()
+    gr.close()
+    for i in range(n_push):
+        assert np.array_equal(host_u16(outs_g[i]), seq[i]), i
+
+
 def test_pipeline_rejects_bad_arguments():
     with pytest.raises(ValueError):
         SlotPipeline(num_ue=4, num_bs=8, num_sc=16, sc_group=4, num_syms=1, mod_order=4,
@@ -147,6 +202,9 @@ def test_pipeline_rejects_bad_arguments():
     with pytest.raises(ValueError):
         SlotPipeline(num_ue=4, num_bs=8, num_sc=16, sc_group=4, num_syms=1, mod_order=4,
                      ue_stride=64, cb_size=64, cb_stride=64, zeros=0, valid_bits=64, depth=1)
+    with pytest.raises(ValueError):
+        SlotPipeline(num_ue=4, num_bs=8, num_sc=16, sc_group=4, num_syms=1, mod_order=4,
+                     ue_stride=64, cb_size=64, cb_stride=64, zeros=0, valid_bits=64, slots=0)
     with pytest.raises(ValueError):
         PipelineGraph([], num_ue=4, num_bs=8, num_sc=16, sc_group=4, num_syms=1, mod_order=4,
                       ue_stride=64, cb_size=64, cb_stride=64, zeros=0, valid_bits=64)
