@@ -23,18 +23,19 @@
 // so D[2r][n] = Re out and D[2r+1][n] = Im out, and each lane ends with
 // whole complex outputs (rows 4*(l/16) .. +3 = two antennas) at column l%16.
 // FP32 accumulation, users in increasing order (two per step).
-#include "internal.h"
+#include "eq_core.h"
 
 namespace mega {
 namespace detail {
 namespace {
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-
 constexpr int kPrecAcc = 8;  // 8-antenna accumulator tiles per wave (64 antennas)
 
-// Workgroup = (group g, block of 64 antennas ab), ab fastest; wave w takes
-// the 16-column tiles w, w + 4, ... A tile's user symbols (at most UMAX users,
+// Workgroup = (group g, block of 64 antennas ab), ab fastest, in the logical
+// order of xcd_linear: consecutive workgroups share an XCD, so a group's user
+// symbols are fetched into one L2 for all its antenna blocks, and adjacent
+// groups' partial 128-byte output lines meet in the same L2. Wave w takes the
+// 16-column tiles w, w + 4, ... A tile's user symbols (at most UMAX users,
 // two per step) are all loaded before its products, so the HBM latency is
 // paid once per tile, not once per step.
 template <int UMAX>
@@ -42,36 +43,39 @@ __global__ __launch_bounds__(256) void precode_mfma(const PrecodeArgs a, uint32_
                                                    uint32_t ntile) {
   constexpr int NKS = UMAX / 2;
   __shared__ float2 sP[kMaxUsers * 64];
-  const uint32_t ab = blockIdx.x % nab, g = blockIdx.x / nab;
+  const uint32_t wg = xcd_linear(blockIdx.x, gridDim.x);
+  const uint32_t ab = wg % nab, g = wg / nab;
   const uint32_t U = a.U, B = a.B, N = a.N, sg = a.sg;
   const uint32_t b0 = ab * 64;
-  const float2* P = a.prec + (uint64_t)g * U * B;
-  for (uint32_t e = threadIdx.x; e < U * 64; e += 256) {
-    const uint32_t u = e / 64, bb = e % 64;
-    sP[e] = b0 + bb < B ? P[(uint64_t)u * B + b0 + bb] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
   const int l = threadIdx.x & 63, i16 = l & 15, q = l >> 4;
   const int up = q >> 1, p = q & 1, h = i16 & 1;
-  // A operand of row 2r + h and K (user, part p): Re P when h == p, else
-  // -Im P (h = 0) or Im P (h = 1)
-  const bool take_re = h == p;
-  const float sgn = h ? 1.f : -1.f;
   const uint32_t ncol = sg * a.S;
-  for (uint32_t t = threadIdx.x >> 6; t < ntile; t += 4) {
-    // B operand: column n of the tile (past the group: its last column,
-    // results dropped), K index q = 2 * (user within the step) + part
-    const uint32_t n_raw = 16 * t + i16;
-    const uint32_t n = min(n_raw, ncol - 1);
+  // B operands of tile t: column n of the tile (past the group: its last
+  // column, results dropped), K index q = 2 * (user within the step) + part
+  auto load_x = [&](uint32_t t, float (&bv)[NKS]) {
+    const uint32_t n = min(16 * t + i16, ncol - 1);
     const uint32_t s = n / sg, k = g * sg + (n - s * sg);
     const float2* xcol = a.ue + (uint64_t)s * a.ue_sym + k;
-    float bv[NKS];
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
       const uint32_t u = 2 * ks + up;
       const float2 x = u < U ? xcol[(uint64_t)u * N] : make_float2(0.f, 0.f);
       bv[ks] = p ? x.y : x.x;
     }
+  };
+  const float2* P = a.prec + (uint64_t)g * U * B;
+  for (uint32_t e = threadIdx.x; e < U * 64; e += 256) {
+    const uint32_t u = e / 64, bb = e % 64;
+    sP[e] = b0 + bb < B ? P[(uint64_t)u * B + b0 + bb] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  // A operand of row 2r + h and K (user, part p): Re P when h == p, else
+  // -Im P (h = 0) or Im P (h = 1)
+  const bool take_re = h == p;
+  const float sgn = h ? 1.f : -1.f;
+  for (uint32_t t = threadIdx.x >> 6; t < ntile; t += 4) {
+    float bv[NKS];
+    load_x(t, bv);
     v4f acc[kPrecAcc];
 #pragma unroll
     for (int m = 0; m < kPrecAcc; ++m) acc[m] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -87,14 +91,17 @@ __global__ __launch_bounds__(256) void precode_mfma(const PrecodeArgs a, uint32_
         acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[ks], acc[m], 0, 0, 0);
       }
     }
-    if (n_raw >= ncol) continue;
-    const uint64_t col = ((uint64_t)k + a.bs_start + a.row / 2) % a.row;
-    float2* O = a.out + (uint64_t)s * a.out_sym + col;
+    const uint32_t n_raw = 16 * t + i16;
+    if (n_raw < ncol) {
+      const uint32_t s = n_raw / sg, k = g * sg + (n_raw - s * sg);
+      const uint64_t col = ((uint64_t)k + a.bs_start + a.row / 2) % a.row;
+      float2* O = a.out + (uint64_t)s * a.out_sym + col;
 #pragma unroll
-    for (int m = 0; m < kPrecAcc; ++m) {
-      const uint32_t b = b0 + 8 * m + 2 * q;
-      if (b < B) O[(uint64_t)b * a.row] = make_float2(acc[m][0], acc[m][1]);
-      if (b + 1 < B) O[(uint64_t)(b + 1) * a.row] = make_float2(acc[m][2], acc[m][3]);
+      for (int m = 0; m < kPrecAcc; ++m) {
+        const uint32_t b = b0 + 8 * m + 2 * q;
+        if (b < B) O[(uint64_t)b * a.row] = make_float2(acc[m][0], acc[m][1]);
+        if (b + 1 < B) O[(uint64_t)(b + 1) * a.row] = make_float2(acc[m][2], acc[m][3]);
+      }
     }
   }
 }
