@@ -159,6 +159,48 @@ typedef struct {
 
 int mega_equalize_slots(const mega_equalize_desc *desc, mega_stream_t stream);
 
+/* ---- Uplink FFT stage: received samples -> y and CSI ------------------------
+ * The reference's UplinkFFT<QuanT> and PilotFFT<QuanT> operators
+ * (src/operators/fft/fft_op.h:52-113, fft_op.cu:143-172: a load kernel,
+ * cuFFT C2C forward and a store kernel) as one fused kernel. For symbol s
+ * and antenna b, x[n] = (re, im) / (max + 1) of the integer pairs at
+ * in + s*in_sym_stride + b*ofdm_ca (quant_bits 16: int16 as `short`, 8:
+ * int8 as `char`), X = forward DFT of size ofdm_ca (unnormalised; a power
+ * of two from 16 to 4096, else MEGA_ERR_UNSUPPORTED), and bin
+ * col(i) = (i + ofdm_start + ofdm_ca/2) % ofdm_ca is subcarrier i < ofdm_num.
+ * The constructor arguments of the reference become descriptor fields. */
+typedef struct {
+  uint32_t ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms;
+  int32_t quant_bits;     /* 16 or 8 */
+  const void *in;         /* [S][B][ofdm_ca] (re, im) integer pairs */
+  uint64_t in_sym_stride; /* samples between symbols (0 = bs_num * ofdm_ca) */
+  void *out;              /* see each call */
+  uint64_t out_sym_stride; /* complex elements between symbols (0 = dense) */
+} mega_fft_desc;
+
+/* UplinkFFT::operator() (fft_op.cu:143-150, fft_store_uplink_kernel
+ * fft_callback.cuh:43-74): out = y, [S][B][ofdm_num] complex64,
+ * y[s][b][i] = X[col(i)] -- the bsVec layout mega_equalize reads. */
+int mega_uplink_fft(const mega_fft_desc *desc, mega_stream_t stream);
+
+/* PilotFFT::operator()(in, out, ue_start) (fft_op.cu:152-172,
+ * fft_store_pilot_kernel fft_callback.cuh:76-108): pilot subcarrier i of
+ * antenna b gives the CSI element of user ue_start + i % sc_group in group
+ * i / sc_group, conj(X[col(i)]) * pilot_sign[i] computed as
+ * {e.re p.re + e.im p.im, e.re p.im - e.im p.re}, written to out =
+ * [G][B][ue_num] complex64 (G = ceil(ofdm_num / sc_group); A = H^H, the
+ * layout mega_zf / mega_beamform_uplink read); users >= ue_num are skipped
+ * and other entries left as they are. pilot_sign: device, ofdm_num complex64
+ * (mega_pilot_sign's table). */
+int mega_pilot_fft(const mega_fft_desc *desc, uint32_t ue_num, uint32_t sc_group,
+                   uint32_t ue_start, const void *pilot_sign, mega_stream_t stream);
+
+/* PilotSign::init's table (fft_op.cu:176-194), on the host: the
+ * Zadoff-Chu sequence of ofdm_num (utils.h:54-75) times the pi/4 cyclic
+ * shift (utils.h:77-95), each element divided by its squared magnitude;
+ * host_out receives ofdm_num (re, im) float pairs. Synchronous, no GPU. */
+int mega_pilot_sign(uint32_t ofdm_num, float *host_out);
+
 /* ---- Downlink precode ----------------------------------------------------- */
 
 /* Replaces the raw overload Precode::precode(problem_size_row = B,

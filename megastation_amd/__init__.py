@@ -99,6 +99,15 @@ class PrecodeDesc(ctypes.Structure):
                 ("bsvec", ctypes.c_void_p), ("bs_sym_stride", ctypes.c_uint64)]
 
 
+class FftDesc(ctypes.Structure):
+    """mega_fft_desc (include/megadetect.h)."""
+    _fields_ = [("ofdm_start", ctypes.c_uint32), ("ofdm_ca", ctypes.c_uint32),
+                ("ofdm_num", ctypes.c_uint32), ("bs_num", ctypes.c_uint32),
+                ("num_syms", ctypes.c_uint32), ("quant_bits", ctypes.c_int32),
+                ("in_", ctypes.c_void_p), ("in_sym_stride", ctypes.c_uint64),
+                ("out", ctypes.c_void_p), ("out_sym_stride", ctypes.c_uint64)]
+
+
 def lib() -> ctypes.CDLL:
     """Load libmegadetect.so (raises if it has not been built)."""
     global _lib
@@ -128,6 +137,9 @@ def lib() -> ctypes.CDLL:
         "mega_detect_work_bytes": ([ctypes.POINTER(DetectDesc)], u64),
         "mega_demod_host": ([P, u64, u8, P], i32),
         "mega_precode": ([u64, u64, u64, P, P, P, u64, u64, u64, P], i32),
+        "mega_uplink_fft": ([ctypes.POINTER(FftDesc), P], i32),
+        "mega_pilot_fft": ([ctypes.POINTER(FftDesc), u32, u32, u32, P, P], i32),
+        "mega_pilot_sign": ([u32, P], i32),
         "mega_precode_syms": ([ctypes.POINTER(PrecodeDesc), P], i32),
         "mega_batch_create": ([ctypes.POINTER(CellDesc), u32, ctypes.c_char_p,
                                ctypes.POINTER(P)], i32),
@@ -298,6 +310,42 @@ def precode(prec, ue, out, *, num_bs, num_ue, num_sc, sc_group, bs_stride, bs_st
                     _ptr(prec).value, _ptr(ue).value, ue_sym_stride or num_ue * num_sc,
                     _ptr(out).value, bs_sym_stride or num_bs * bs_stride)
     _check(lib().mega_precode_syms(ctypes.byref(d), _stream(stream)))
+
+
+def _fft_desc(iq, out, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms, in_sym_stride,
+              out_sym_stride):
+    import torch
+    bits = {torch.int16: 16, torch.int8: 8}.get(getattr(iq, "dtype", None))
+    if bits is None:
+        raise ValueError("samples must be an int16 or int8 tensor of (re, im) pairs")
+    return FftDesc(ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms, bits, _ptr(iq).value,
+                   in_sym_stride, _ptr(out).value, out_sym_stride)
+
+
+def uplink_fft(iq, y, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms: int = 1,
+               in_sym_stride: int = 0, out_sym_stride: int = 0, stream=None):
+    """UplinkFFT (mega_uplink_fft): iq [S][B][ca][2] int16/int8 -> y
+    [S][B][ofdm_num] complex64."""
+    d = _fft_desc(iq, y, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms, in_sym_stride,
+                  out_sym_stride)
+    _check(lib().mega_uplink_fft(ctypes.byref(d), _stream(stream)))
+
+
+def pilot_fft(iq, csi, pilot, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, ue_num, sc_group,
+              ue_start: int = 0, stream=None):
+    """PilotFFT (mega_pilot_fft): one pilot symbol iq [B][ca][2] -> CSI
+    [G][B][U] complex64 (A = H^H); pilot: device complex64 [ofdm_num]."""
+    d = _fft_desc(iq, csi, ofdm_start, ofdm_ca, ofdm_num, bs_num, 1, 0, 0)
+    _check(lib().mega_pilot_fft(ctypes.byref(d), ue_num, sc_group, ue_start, _ptr(pilot),
+                                _stream(stream)))
+
+
+def pilot_sign(ofdm_num: int):
+    """PilotSign::init's table (mega_pilot_sign, host): numpy complex64."""
+    import numpy as np
+    out = np.zeros(2 * ofdm_num, np.float32)
+    _check(lib().mega_pilot_sign(ofdm_num, out.ctypes.data_as(ctypes.c_void_p)))
+    return out.view(np.complex64)
 
 
 def detect_fused_shape(num_ue: int, num_bs: int) -> bool:

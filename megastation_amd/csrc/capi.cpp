@@ -3,6 +3,7 @@
 // the reference interface it replaces.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -305,6 +306,95 @@ int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
   d.y = bsvec;
   d.llr = uevec;
   return mega_equalize_slots(&d, stream);
+}
+
+static int fft_checked(const mega_fft_desc* d, bool pilot, uint32_t ue_num, uint32_t sc_group,
+                       uint32_t ue_start, const void* pilot_sign, mega_stream_t stream,
+                       const char* who) {
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": desc is NULL");
+  if (d->bs_num == 0 || d->num_syms == 0 || d->ofdm_num == 0) return MEGA_OK;
+  if (!d->in || !d->out) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
+  if (pilot && !pilot_sign)
+    return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL pilot sign table");
+  if (pilot && (sc_group == 0 || ue_num == 0))
+    return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": zero users or group size");
+  mega::detail::FftArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.in = d->in;
+  a.out = static_cast<float2*>(d->out);
+  a.pilot = static_cast<const float2*>(pilot_sign);
+  a.B = d->bs_num;
+  a.S = d->num_syms;
+  a.ca = d->ofdm_ca;
+  a.num = d->ofdm_num;
+  a.start = d->ofdm_start;
+  a.qbits = (uint32_t)d->quant_bits;
+  a.U = ue_num;
+  a.sg = sc_group;
+  a.ue_start = ue_start;
+  a.in_sym = d->in_sym_stride ? d->in_sym_stride : (uint64_t)d->bs_num * d->ofdm_ca;
+  a.out_sym = d->out_sym_stride ? d->out_sym_stride
+              : pilot ? (uint64_t)((d->ofdm_num + sc_group - 1) / sc_group) * d->bs_num * ue_num
+                      : (uint64_t)d->bs_num * d->ofdm_num;
+  const char* why = nullptr;
+  hipError_t e = mega::detail::launch_fft(a, pilot, static_cast<hipStream_t>(stream), &why);
+  if (e == hipErrorInvalidValue && why) return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": " + why);
+  if (e != hipSuccess) return hip_fail(e, who, why);
+  return MEGA_OK;
+}
+
+int mega_uplink_fft(const mega_fft_desc* d, mega_stream_t stream) {
+  return fft_checked(d, false, 0, 0, 0, nullptr, stream, "mega_uplink_fft");
+}
+
+int mega_pilot_fft(const mega_fft_desc* d, uint32_t ue_num, uint32_t sc_group, uint32_t ue_start,
+                   const void* pilot_sign, mega_stream_t stream) {
+  return fft_checked(d, true, ue_num, sc_group, ue_start, pilot_sign, stream, "mega_pilot_fft");
+}
+
+// Largest prime <= n among the primes up to 2039, else 2039: the length rule
+// of the reference's Zadoff-Chu generator (utils.h:54-64), whose prime table
+// ends at 2039.
+static uint32_t zc_length(uint64_t n) {
+  uint32_t best = 0;
+  for (uint32_t p = 2; p <= 2039; ++p) {
+    bool prime = true;
+    for (uint32_t d = 2; d * d <= p; ++d)
+      if (p % d == 0) {
+        prime = false;
+        break;
+      }
+    if (!prime) continue;
+    if (p <= n) best = p;
+    else break;
+  }
+  // n >= 2039 or no prime <= n (n < 2): the table's last entry
+  return (best == 0 || n >= 2039) ? 2039 : best;
+}
+
+int mega_pilot_sign(uint32_t ofdm_num, float* host_out) {
+  // every product and sum rounded on its own, as the reference's host code
+#pragma clang fp contract(off)
+  if (!host_out && ofdm_num) return fail(MEGA_ERR_INVALID_ARG, "mega_pilot_sign: NULL output");
+  const uint32_t L = zc_length(ofdm_num);
+  const double r = std::floor(L * 2.0 / 31.0 + 0.5);
+  const float alpha = (float)(M_PI / 4);
+  for (uint32_t i = 0; i < ofdm_num; ++i) {
+    // Zadoff-Chu root r (utils.h:66-72): the phase formed in double, its
+    // sine and cosine in float
+    const uint32_t m = i % L;
+    float zs, zc;
+    sincosf((float)(-M_PI * r * m * (m + 1) / L), &zs, &zc);
+    // cyclic shift by alpha * i (utils.h:77-95), product as mega::Complex
+    float cs, cc;
+    sincosf(alpha * (float)i, &cs, &cc);
+    const float re = zc * cc - zs * cs, im = zc * cs + zs * cc;
+    // divided by its squared magnitude (fft_op.cu:183-186)
+    const float d = re * re + im * im;
+    host_out[2 * i] = re / d;
+    host_out[2 * i + 1] = im / d;
+  }
+  return MEGA_OK;
 }
 
 int mega_precode_syms(const mega_precode_desc* d, mega_stream_t stream) {

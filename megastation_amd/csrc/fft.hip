@@ -1,0 +1,119 @@
+// Uplink FFT stage of the detection path: the reference's UplinkFFT and
+// PilotFFT operators (src/operators/fft/fft_op.cu:143-172; cuFFT C2C forward
+// between a load and a store kernel, fft_callback.cuh:20-108), fused here
+// into one kernel per launch. For OFDM symbol s of antenna b:
+//   x[n]  = (re, im) / (max + 1) of the received integer pairs (int16/int8),
+//   X[k]  = sum_n x[n] exp(-2 pi i k n / ca)           (unnormalised),
+// then either
+//   uplink: y[s][b][i] = X[(i + start + ca/2) % ca], i < num  (the [B][N]
+//           layout the equalize reads), or
+//   pilot:  the CSI element of user ue_start + i % sg in group i / sg,
+//           conj(X[col(i)]) * pilot[i], into [G][B][U] (A = H^H, the layout
+//           the ZF kernels read); users >= U are skipped.
+//
+// One workgroup of 256 threads per (antenna, symbol): samples are loaded
+// coalesced and scaled into LDS in bit-reversed order, ca/2 twiddles are
+// computed once into LDS (sincospif of exact arguments), and log2(ca)
+// radix-2 decimation-in-time stages run in place with a barrier between
+// them; the store reads the shifted bins straight from LDS. Sizes: powers of
+// two from 16 to 4096 (48 KB of LDS at 4096).
+#include "internal.h"
+
+namespace mega {
+namespace detail {
+namespace {
+
+constexpr int kFftThreads = 256;
+constexpr uint32_t kFftMax = 4096;
+
+template <typename Q>
+__device__ __forceinline__ float2 load_iq(const Q* p, uint64_t n, float inv) {
+  return make_float2((float)p[2 * n] * inv, (float)p[2 * n + 1] * inv);
+}
+
+template <typename Q, bool PILOT>
+__global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
+  __shared__ float2 sx[kFftMax];
+  __shared__ float2 tw[kFftMax / 2];
+  const uint32_t b = blockIdx.x, s = blockIdx.y;
+  const uint32_t N = a.ca, L = a.log2ca, H = N / 2;
+  const Q* in = static_cast<const Q*>(a.in) + 2 * ((uint64_t)s * a.in_sym + (uint64_t)b * N);
+  const float inv = 1.0f / a.qscale;
+  for (uint32_t n = threadIdx.x; n < N; n += kFftThreads) {
+    const uint32_t r = __builtin_bitreverse32(n) >> (32 - L);
+    sx[r] = load_iq(in, n, inv);
+  }
+  for (uint32_t k = threadIdx.x; k < H; k += kFftThreads) {
+    float sn, cs;
+    sincospif(-2.0f * (float)k / (float)N, &sn, &cs);  // exp(-2 pi i k / N)
+    tw[k] = make_float2(cs, sn);
+  }
+  __syncthreads();
+  for (uint32_t st = 0; st < L; ++st) {
+    const uint32_t half = 1u << st;
+    for (uint32_t j = threadIdx.x; j < H; j += kFftThreads) {
+      const uint32_t k = j & (half - 1);
+      const uint32_t i0 = ((j >> st) << (st + 1)) + k, i1 = i0 + half;
+      const float2 w = tw[k << (L - 1 - st)];
+      const float2 u = sx[i0], v = sx[i1];
+      const float2 t = make_float2(w.x * v.x - w.y * v.y, w.x * v.y + w.y * v.x);
+      sx[i0] = make_float2(u.x + t.x, u.y + t.y);
+      sx[i1] = make_float2(u.x - t.x, u.y - t.y);
+    }
+    __syncthreads();
+  }
+  const uint32_t shift = a.start + H;
+  if constexpr (!PILOT) {
+    float2* y = a.out + (uint64_t)s * a.out_sym + (uint64_t)b * a.num;
+    for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) y[i] = sx[(i + shift) % N];
+  } else {
+    for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) {
+      const uint32_t u = a.ue_start + i % a.sg;
+      if (u >= a.U) continue;
+      const float2 e = sx[(i + shift) % N], p = a.pilot[i];
+      const uint64_t o = (uint64_t)(i / a.sg) * a.B * a.U + (uint64_t)b * a.U + u;
+      a.out[(uint64_t)s * a.out_sym + o] =
+          make_float2(e.x * p.x + e.y * p.y, e.x * p.y - e.y * p.x);
+    }
+  }
+}
+
+template <bool PILOT>
+hipError_t launch_q(const FftArgs& a, hipStream_t st) {
+  const dim3 grid(a.B, a.S), block(kFftThreads);
+  if (a.qbits == 16)
+    hipLaunchKernelGGL((fft_kernel<int16_t, PILOT>), grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL((fft_kernel<int8_t, PILOT>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fft(const FftArgs& a0, bool pilot, hipStream_t st, const char** why) {
+  FftArgs a = a0;
+  if (a.B == 0 || a.S == 0 || a.num == 0) return hipSuccess;
+  if (a.ca < 16 || a.ca > kFftMax || (a.ca & (a.ca - 1)) != 0) {
+    *why = "ofdm_ca must be a power of two from 16 to 4096";
+    return hipErrorInvalidValue;
+  }
+  if (a.num > a.ca || a.start >= a.ca) {
+    *why = "ofdm_num and ofdm_start must be below ofdm_ca";
+    return hipErrorInvalidValue;
+  }
+  if (a.qbits != 16 && a.qbits != 8) {
+    *why = "samples must be 16- or 8-bit integer pairs";
+    return hipErrorInvalidValue;
+  }
+  if (a.B > 0x7fffffffu || a.S > 65535) {
+    *why = "grid too large";
+    return hipErrorInvalidValue;
+  }
+  if (pilot && (a.sg == 0 || a.U == 0)) return hipSuccess;
+  a.log2ca = 31 - __builtin_clz(a.ca);
+  a.qscale = a.qbits == 16 ? 32768.0f : 128.0f;
+  return pilot ? launch_q<true>(a, st) : launch_q<false>(a, st);
+}
+
+}  // namespace detail
+}  // namespace mega

@@ -84,6 +84,21 @@ struct PrecodeArgs {
 };
 hipError_t launch_precode(const PrecodeArgs& a, hipStream_t stream, const char** why);
 
+// Uplink / pilot FFT of S symbols x B antennas (fft.hip): in [S][B][ca]
+// integer (re, im) pairs (in_sym samples apart), out y [S][B][num] or CSI
+// [G][B][U] (out_sym complex elements apart).
+struct FftArgs {
+  const void* in;
+  float2* out;
+  const float2* pilot;  // pilot only: [num]
+  uint32_t B, S, ca, num, start, qbits;
+  uint32_t U, sg, ue_start;  // pilot only
+  uint64_t in_sym, out_sym;
+  uint32_t log2ca;  // set by the launcher
+  float qscale;     // set by the launcher
+};
+hipError_t launch_fft(const FftArgs& a, bool pilot, hipStream_t stream, const char** why);
+
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,
                                hipStream_t stream);

@@ -1,0 +1,137 @@
+"""Oracle of the uplink FFT stage (CSI producer): numpy restatement.
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels of
+megastation_amd (mega_uplink_fft, mega_pilot_fft, mega_pilot_sign); never
+imported by the product.
+
+Follows src/operators/fft/fft_callback.cuh and fft_op.cu (UplinkFFT /
+PilotFFT, cuFFT C2C forward, unnormalised) and src/operators/utils.h
+(zadoff_chu_sequence, cyclic_shift); PilotSign::init is fft_op.cu:176-194.
+The transform itself is numpy's FFT in complex128 (the textbook DFT
+X[k] = sum_n x[n] exp(-2 pi i k n / N)); everything around it in FP32 as
+the reference computes it. Pinned by the reference's own fixtures
+(tests/test_oracle_fft.py): test/fft/fft_out.data, complex_out.data,
+zachu.data, zachu_shift.data.
+"""
+from __future__ import annotations
+
+import ctypes
+import ctypes.util
+import math
+
+import numpy as np
+
+# The reference computes these tables on the host with the C library's
+# sincosf (float); the same function is called here, so they match bit for
+# bit on the same libm.
+_libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+_libm.sincosf.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                          ctypes.POINTER(ctypes.c_float)]
+_libm.sincosf.restype = None
+
+
+def _sincosf(x: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    s, c = ctypes.c_float(), ctypes.c_float()
+    sn = np.empty(len(x), np.float32)
+    cs = np.empty(len(x), np.float32)
+    for i, v in enumerate(np.asarray(x, np.float32)):
+        _libm.sincosf(ctypes.c_float(v), ctypes.byref(s), ctypes.byref(c))
+        sn[i], cs[i] = s.value, c.value
+    return sn, cs
+
+ZC_MAX_PRIME = 2039  # the reference's prime table ends here (utils.h:25-52)
+
+
+def _primes_upto(n: int) -> list[int]:
+    sieve = np.ones(n + 1, dtype=bool)
+    sieve[:2] = False
+    for i in range(2, int(n ** 0.5) + 1):
+        if sieve[i]:
+            sieve[i * i::i] = False
+    return [int(p) for p in np.nonzero(sieve)[0]]
+
+
+def zc_length(seq_len: int) -> int:
+    """Largest prime <= seq_len among the primes up to 2039; 2039 when
+    seq_len is at least 2039 or below 2 (utils.h:58-64)."""
+    ps = _primes_upto(ZC_MAX_PRIME)
+    for j in range(len(ps) - 1):
+        if ps[j] <= seq_len < ps[j + 1]:
+            return ps[j]
+    return ZC_MAX_PRIME
+
+
+def zadoff_chu(seq_len: int) -> np.ndarray:
+    """utils.h:54-75: x[i] = exp(-i pi r m (m+1) / L), m = i mod L, r =
+    floor(2L/31 + 0.5); the phase is formed in double and rounded to float
+    (the argument of sincosf), the sine and cosine taken in float."""
+    L = zc_length(seq_len)
+    r = math.floor(L * 2.0 / 31.0 + 0.5)
+    m = np.arange(seq_len) % L
+    arg = (-math.pi * r * m.astype(np.float64) * (m + 1) / L).astype(np.float32)
+    sn, cs = _sincosf(arg)
+    return (cs + 1j * sn).astype(np.complex64)
+
+
+def cyclic_shift(seq: np.ndarray, alpha: float) -> np.ndarray:
+    """utils.h:77-95: seq[i] * exp(i alpha i), alpha * i in float; the
+    product in the order of mega::Complex."""
+    a = np.float32(alpha)
+    ph = (a * np.arange(len(seq), dtype=np.float32)).astype(np.float32)
+    ci, cr = _sincosf(ph)
+    sr, si = seq.real.astype(np.float32), seq.imag.astype(np.float32)
+    re = sr * cr - si * ci
+    im = sr * ci + si * cr
+    return (re + 1j * im).astype(np.complex64)
+
+
+def pilot_sign(ofdm_num: int) -> np.ndarray:
+    """PilotSign::init (fft_op.cu:176-194): zadoff_chu(n) shifted by pi/4,
+    each element divided by its squared magnitude (in float)."""
+    p = cyclic_shift(zadoff_chu(ofdm_num), math.pi / 4)
+    re, im = p.real.astype(np.float32), p.imag.astype(np.float32)
+    d = re * re + im * im
+    return ((re / d) + 1j * (im / d)).astype(np.complex64)
+
+
+def _load(iq: np.ndarray, bs: int, ca: int) -> np.ndarray:
+    """fft_load_kernel (fft_callback.cuh:20-40): q / (max + 1) in float."""
+    scale = np.float32(np.iinfo(iq.dtype).max + 1.0)
+    x = iq.reshape(bs, ca, 2).astype(np.float32) / scale
+    return x[..., 0].astype(np.float64) + 1j * x[..., 1].astype(np.float64)
+
+
+def _spectrum_cols(ofdm_start: int, ca: int, num: int) -> np.ndarray:
+    return (np.arange(num) + ofdm_start + ca // 2) % ca
+
+
+def uplink_fft(iq: np.ndarray, ofdm_start: int, ca: int, num: int, bs: int) -> np.ndarray:
+    """UplinkFFT (fft_op.cu:143-150, fft_store_uplink_kernel :43-74): iq is
+    [bs][ca] (re, im) integer pairs; returns y [bs][num] complex64 with
+    y[b][i] = X_b[(i + ofdm_start + ca/2) % ca]."""
+    X = np.fft.fft(_load(iq, bs, ca), axis=1)
+    return X[:, _spectrum_cols(ofdm_start, ca, num)].astype(np.complex64)
+
+
+def pilot_fft(iq: np.ndarray, ofdm_start: int, ca: int, num: int, bs: int, ue: int,
+              sg: int, pilot: np.ndarray, ue_start: int = 0,
+              out: np.ndarray | None = None) -> np.ndarray:
+    """PilotFFT (fft_op.cu:152-172, fft_store_pilot_kernel :76-108): the
+    pilot subcarrier i of antenna b gives the CSI element of user
+    ue_start + i % sg in group i / sg, conj(X_b[col(i)]) * pilot[i] written
+    as {e.re p.re + e.im p.im, e.re p.im - e.im p.re}, into [G][bs][ue]
+    (users >= ue skipped; other entries of `out` kept)."""
+    X = np.fft.fft(_load(iq, bs, ca), axis=1)[:, _spectrum_cols(ofdm_start, ca, num)]
+    X = X.astype(np.complex64)
+    G = (num + sg - 1) // sg
+    o = np.zeros((G, bs, ue), np.complex64) if out is None else out.reshape(G, bs, ue).copy()
+    er, ei = X.real.astype(np.float32), X.imag.astype(np.float32)
+    pr, pi = pilot[:num].real.astype(np.float32), pilot[:num].imag.astype(np.float32)
+    re = er * pr + ei * pi
+    im = er * pi - ei * pr
+    for i in range(num):
+        u = ue_start + i % sg
+        if u >= ue:
+            continue
+        o[i // sg, :, u] = re[:, i] + 1j * im[:, i]
+    return o

@@ -17,6 +17,17 @@ Sources (binary fixtures next to the reference tests):
                             transposes them to [ue][ofdm] before upload
   test/sgemv/precoded.data  expected output, [bs=8][ofdmE=32] complex64, for
                             precode_skip = 16 and bs_start = 8 (1e-5)
+  test/fft/fft_in.data      uplink FFT input, [bs=8][32] int16 (re, im) pairs
+                            (test/fft/test_fft.cc:105-160)
+  test/fft/fft_out.data     expected uplink output, [16][bs=8] complex64 for
+                            ofdm_start 8, ofdm_ca 32, ofdm_num 16 (1e-5)
+  test/fft/short_in.data    pilot FFT input, [bs=8][32] int16 pairs (:56-103)
+  test/fft/complex_out.data expected CSI, [G=4][bs=8][ue=4] complex64 for
+                            ofdm_start 8, ofdm_ca 32, 16 subcarriers, groups
+                            of 4, pilot sign of 16 (1e-4)
+  test/fft/zachu.data       Zadoff-Chu sequence of length 32, first 31 values
+                            (complex64, 1e-5; :10-30)
+  test/fft/zachu_shift.data Zadoff-Chu of 16 times the pi/4 cyclic shift (:32-54)
 
 The arrays are stored unchanged (raw little-endian values); this script only
 re-packages them so the tests run where the reference tree is absent.
@@ -47,6 +58,12 @@ def main():
         "precode_prec": rd("test/sgemv/prec.data", np.complex64),
         "precode_uevec_ku": rd("test/sgemv/ueVec.data", np.complex64),
         "precode_out": rd("test/sgemv/precoded.data", np.complex64),
+        "fft_uplink_in": rd("test/fft/fft_in.data", np.int16),
+        "fft_uplink_out_kb": rd("test/fft/fft_out.data", np.complex64),
+        "fft_pilot_in": rd("test/fft/short_in.data", np.int16),
+        "fft_pilot_csi": rd("test/fft/complex_out.data", np.complex64),
+        "zadoff_chu_32": rd("test/fft/zachu.data", np.complex64),
+        "zadoff_chu_16_shift": rd("test/fft/zachu_shift.data", np.complex64),
     }
     assert arrays["sgemv_csi"].size == 4 * 8
     assert arrays["sgemv_bsvec_kb"].size == 16 * 8
@@ -55,6 +72,12 @@ def main():
     assert arrays["precode_prec"].size == 8 * 4
     assert arrays["precode_uevec_ku"].size == 16 * 4
     assert arrays["precode_out"].size == 8 * 32
+    assert arrays["fft_uplink_in"].size == 2 * 8 * 32
+    assert arrays["fft_uplink_out_kb"].size == 16 * 8
+    assert arrays["fft_pilot_in"].size == 2 * 8 * 32
+    assert arrays["fft_pilot_csi"].size == 4 * 8 * 4
+    assert arrays["zadoff_chu_32"].size == 31
+    assert arrays["zadoff_chu_16_shift"].size == 16
     np.savez(OUT, **arrays)
     print("wrote", OUT)
 

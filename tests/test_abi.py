@@ -100,6 +100,37 @@ def test_bad_precode_arguments():
     assert ctypes.sizeof(m.PrecodeDesc) == 80
 
 
+@pytest.mark.parametrize("n", [16, 32, 1200, 3276])
+def test_pilot_sign_matches_oracle_bit_for_bit(n):
+    """mega_pilot_sign (host) against the oracle's restatement of
+    PilotSign::init on the same libm sincosf."""
+    from oracle import fft as offt
+    a = m.pilot_sign(n)
+    b = offt.pilot_sign(n)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_bad_fft_arguments():
+    L = m.lib()
+    d = m.FftDesc(8, 32, 16, 8, 1, 16, 1, 0, 1, 0)
+    assert L.mega_uplink_fft(None, None) == m.MEGA_ERR_INVALID_ARG
+    d.ofdm_ca = 48  # not a power of two
+    assert L.mega_uplink_fft(ctypes.byref(d), None) == m.MEGA_ERR_UNSUPPORTED
+    d.ofdm_ca = 8192  # above 4096
+    assert L.mega_uplink_fft(ctypes.byref(d), None) == m.MEGA_ERR_UNSUPPORTED
+    d.ofdm_ca, d.ofdm_num = 32, 40  # more subcarriers than bins
+    assert L.mega_uplink_fft(ctypes.byref(d), None) == m.MEGA_ERR_UNSUPPORTED
+    d.ofdm_num, d.quant_bits = 16, 12
+    assert L.mega_uplink_fft(ctypes.byref(d), None) == m.MEGA_ERR_UNSUPPORTED
+    d.quant_bits, d.in_ = 16, None
+    assert L.mega_uplink_fft(ctypes.byref(d), None) == m.MEGA_ERR_INVALID_ARG
+    d.in_ = 1
+    assert L.mega_pilot_fft(ctypes.byref(d), 4, 4, 0, None, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_pilot_fft(ctypes.byref(d), 4, 0, 0, 1, None) == m.MEGA_ERR_INVALID_ARG
+    assert L.mega_pilot_sign(4, None) == m.MEGA_ERR_INVALID_ARG
+    assert ctypes.sizeof(m.FftDesc) == 56
+
+
 def test_equalize_desc_layout_matches_header():
     # offsets of the C struct on LP64 (natural alignment)
     D = m.EqualizeDesc

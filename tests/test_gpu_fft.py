@@ -1,0 +1,97 @@
+"""The fused uplink FFT stage on cuda:0 (mega_uplink_fft, mega_pilot_fft)
+against the reference's FFT test fixtures and the oracle (oracle/fft.py:
+numpy's DFT in complex128 around the reference's FP32 load and store). FP32
+radix-2 transforms: the bar is the reference tests' 1e-5 (1e-4 for the pilot
+product), relative to the spectrum's scale at the larger sizes. Outputs are
+prefilled with a sentinel, so entries the reference leaves untouched (users
+past U in the CSI, stride padding) are checked too."""
+import numpy as np
+import pytest
+
+import megastation_amd as m
+from oracle import fft as offt
+
+from gpu_helpers import dev_c64, host_c64, torch
+
+pytestmark = pytest.mark.gpu
+
+SENT = np.complex64(7 - 3j)
+
+
+def _iq(t, a):
+    return t.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def test_uplink_fft_reference_fixture(golden):
+    print("case: uplink FFT fixture 8 antennas, 32 bins, 16 subcarriers", flush=True)
+    t = torch()
+    y = t.zeros(8 * 16, dtype=t.complex64, device="cuda:0")
+    m.uplink_fft(_iq(t, golden["fft_uplink_in"]), y, ofdm_start=8, ofdm_ca=32, ofdm_num=16,
+                 bs_num=8)
+    t.cuda.synchronize()
+    ref = golden["fft_uplink_out_kb"].reshape(16, 8).T
+    assert np.abs(host_c64(y).reshape(8, 16) - ref).max() < 1e-5
+
+
+def test_pilot_fft_reference_fixture(golden):
+    print("case: pilot FFT fixture 8 antennas, 4 users, groups of 4", flush=True)
+    t = torch()
+    p = dev_c64(m.pilot_sign(16))
+    csi = t.zeros(4 * 8 * 4, dtype=t.complex64, device="cuda:0")
+    m.pilot_fft(_iq(t, golden["fft_pilot_in"]), csi, p, ofdm_start=8, ofdm_ca=32, ofdm_num=16,
+                bs_num=8, ue_num=4, sc_group=4)
+    t.cuda.synchronize()
+    assert np.abs(host_c64(csi) - golden["fft_pilot_csi"]).max() < 1e-4
+
+
+@pytest.mark.parametrize("B,ca,num,start,S,dtype", [
+    (64, 2048, 1200, 424, 2, np.int16),   # 64-antenna cell, 2048-point
+    (256, 4096, 3276, 410, 3, np.int16),  # 256-antenna cell, 4096-point
+    (12, 512, 300, 100, 1, np.int8),      # char samples
+    (5, 16, 16, 15, 2, np.int16),         # smallest size, every bin
+])
+def test_uplink_fft_matches_oracle(B, ca, num, start, S, dtype):
+    print(f"case: uplink FFT B={B} ca={ca} num={num} start={start} S={S} {dtype.__name__}",
+          flush=True)
+    t = torch()
+    rng = np.random.default_rng(ca + B)
+    hi = np.iinfo(dtype).max
+    iq = rng.integers(-hi // 4, hi // 4, size=(S, B, ca, 2)).astype(dtype)
+    pad = 7
+    y = t.from_numpy(np.full(S * (B * num + pad), SENT, np.complex64)).to("cuda:0")
+    m.uplink_fft(_iq(t, iq), y, ofdm_start=start, ofdm_ca=ca, ofdm_num=num, bs_num=B,
+                 num_syms=S, out_sym_stride=B * num + pad)
+    t.cuda.synchronize()
+    got = host_c64(y).reshape(S, B * num + pad)
+    for s in range(S):
+        exp = offt.uplink_fft(iq[s], start, ca, num, B)
+        scale = float(np.abs(exp).max())
+        np.testing.assert_allclose(got[s, :B * num].reshape(B, num), exp, rtol=0,
+                                   atol=1e-5 * max(1.0, scale))
+        assert (got[s, B * num:] == SENT).all()
+
+
+@pytest.mark.parametrize("B,ca,num,start,U,sg,ue_start", [
+    (64, 2048, 1200, 424, 16, 16, 0),
+    (256, 4096, 3276, 410, 32, 12, 0),
+    (16, 256, 200, 30, 6, 8, 3),  # users past U skipped, groups of 8, offset
+])
+def test_pilot_fft_matches_oracle(B, ca, num, start, U, sg, ue_start):
+    print(f"case: pilot FFT B={B} ca={ca} num={num} U={U} sg={sg} ue_start={ue_start}",
+          flush=True)
+    t = torch()
+    rng = np.random.default_rng(num + U)
+    iq = rng.integers(-8000, 8000, size=(B, ca, 2)).astype(np.int16)
+    p = m.pilot_sign(num)
+    G = (num + sg - 1) // sg
+    csi = t.from_numpy(np.full(G * B * U, SENT, np.complex64)).to("cuda:0")
+    m.pilot_fft(_iq(t, iq), csi, dev_c64(p), ofdm_start=start, ofdm_ca=ca, ofdm_num=num,
+                bs_num=B, ue_num=U, sc_group=sg, ue_start=ue_start)
+    t.cuda.synchronize()
+    fill = np.full((G, B, U), SENT, np.complex64)
+    exp = offt.pilot_fft(iq, start, ca, num, B, U, sg, p, ue_start=ue_start, out=fill)
+    got = host_c64(csi).reshape(G, B, U)
+    untouched = exp == SENT
+    assert np.array_equal(got[untouched], exp[untouched])
+    scale = float(np.abs(exp[~untouched]).max())
+    np.testing.assert_allclose(got, exp, rtol=0, atol=1e-5 * max(1.0, scale))
