@@ -112,7 +112,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2_64x16_200slots",
                     choices=sorted(WORKLOADS) + ["c4_40cells", "c5_learned",
-                                                 "d3_precode_256x32"])
+                                                 "d3_precode_256x32", "u3_fft_256x32"])
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -126,6 +126,8 @@ def main():
         return main_learned(args)
     if args.workload == "d3_precode_256x32" and args.gpus == 1:
         return main_precode(args)
+    if args.workload == "u3_fft_256x32" and args.gpus == 1:
+        return main_fft(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the detection path shards by subcarrier group with no exchange; "
@@ -660,6 +662,100 @@ def main_precode(args):
                                   "kind": "port",
                                   "sample": "1 of 13 symbols, median of 3, oracle C port "
                                             "(single thread)"}
+    print(json.dumps(result), flush=True)
+
+
+def main_fft(args):
+    """Uplink FFT stage of a 256x32 cell (mega_pilot_fft, mega_uplink_fft):
+    4096-point symbols of 16-bit (re, im) samples, 3276 subcarriers from bin
+    offset 410; one step = one slot: three pilot symbols (32 users in groups
+    of 12, users 0-11, 12-23, 24-31) into the CSI [273][256][32], and 13
+    data symbols into y [13][256][3276] in one launch. Four rotating buffer
+    sets. The CPU baseline is numpy's FFT (the oracle's) on one symbol."""
+    import torch
+    import megastation_amd as m
+    B, U, ca, N, sg, S, P, start = 256, 32, 4096, 3276, 12, 13, 3, 410
+    G = (N + sg - 1) // sg
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(99)
+    NR = 4
+
+    def samples(n):
+        return torch.randint(-8000, 8000, (n,), dtype=torch.int16, device=dev, generator=gen)
+    D = [samples(S * B * ca * 2) for _ in range(NR)]
+    PL = [samples(P * B * ca * 2) for _ in range(NR)]
+    Y = [torch.empty(S * B * N, dtype=torch.complex64, device=dev) for _ in range(NR)]
+    C = [torch.empty(G * B * U, dtype=torch.complex64, device=dev) for _ in range(NR)]
+    psign = torch.from_numpy(m.pilot_sign(N)).to(dev)
+    stream = torch.cuda.current_stream()
+    kw = dict(ofdm_start=start, ofdm_ca=ca, ofdm_num=N, bs_num=B, stream=stream)
+
+    def step(j, evs=None):
+        j %= NR
+        if evs:
+            evs[0].record(stream)
+        for p in range(P):
+            m.pilot_fft(PL[j][p * B * ca * 2:], C[j], psign, ue_num=U, sc_group=sg,
+                        ue_start=p * sg, **kw)
+        if evs:
+            evs[1].record(stream)
+        m.uplink_fft(D[j], Y[j], num_syms=S, **kw)
+        if evs:
+            evs[2].record(stream)
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    for j in range(args.warmup):
+        step(j)
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for j in range(args.steps):
+        step(j, evs[j])
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / args.steps
+    pil_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
+    up_ms = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
+    sym = B * (S + P)  # antenna-symbols transformed per step
+    up_bytes = S * B * (4 * ca + 8 * N)
+    result = {
+        "metric": "antenna-symbols/s through the uplink FFT stage (pilot CSI + data y), 1 GPU",
+        "value": sym / (ms * 1e-3), "unit": "antenna-symbols/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (int16 samples)", "data": "synthetic",
+        "config": {"workload": "u3_fft_256x32", "antennas": B, "users": U, "fft_size": ca,
+                   "subcarriers": N, "sc_group": sg, "data_symbols": S, "pilot_symbols": P,
+                   "ofdm_start": start, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": "fft_kernel (uplink, 13 symbols)",
+                     "achieved": up_bytes / (up_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": up_bytes / (up_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel") or
+                                 {}).get("traffic_bytes"),
+                     "algorithmic_bytes": up_bytes, "avg_ms": up_ms},
+        "forms": {"pilot_ms": pil_ms, "uplink_ms": up_ms},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import fft as offt
+        iq = D[0][:B * ca * 2].cpu().numpy()
+        ts = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            offt.uplink_fft(iq, start, ca, N, B)
+            ts.append(time.perf_counter() - c0)
+        result["cpu_baseline"] = {"value": B / statistics.median(ts), "unit": "antenna-symbols/s",
+                                  "cores": 1, "kind": "port",
+                                  "sample": "1 of 13 data symbols (256 antennas), median of 3, "
+                                            "numpy FFT (the oracle)"}
     print(json.dumps(result), flush=True)
 
 

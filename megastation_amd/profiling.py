@@ -64,7 +64,7 @@ def traffic(fetch_dir: str, write_dir: str, copy_bytes: int) -> dict:
     res = {"copy_bytes": copy_bytes, "copy_fetch_kib": copy_f, "copy_write_kib": copy_w,
            "fetch_scale": kf, "write_scale": kw, "kernels": {}}
     for (k, c), vals in f.items():
-        if not k.startswith(("equalize", "zf", "detect", "precode")):
+        if not k.startswith(("equalize", "zf", "detect", "precode", "fft")):
             continue
         wv = w.get((k, "WRITE_SIZE"), [0.0])
         fb = statistics.median(vals) * 1024 * kf

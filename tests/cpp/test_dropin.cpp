@@ -3,12 +3,12 @@
 //
 //   test_dropin host            descriptor tests + demod known answer (no GPU)
 //   test_dropin gpu <dir>       all of the above with device buffers, plus the
-//                               equalize, precode and beamform fixtures read
-//                               from <dir>
+//                               equalize, precode, FFT and beamform fixtures
+//                               read from <dir>
 //
 // Mirrors test/matrix/test_matrix.cc, test/matrix/test_cuphy_tensor.cc,
-// test/modulation/test_modulation.cc:11-57, test/sgemv/test_sgemv.cc:126-196
-// and test/beamform/test_beamform.cc:67-95.
+// test/modulation/test_modulation.cc:11-57, test/sgemv/test_sgemv.cc:126-196,
+// test/fft/test_fft.cc:56-160 and test/beamform/test_beamform.cc:67-95.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,6 +22,7 @@
 #include "mega/beamform.h"
 #include "mega/demodulation.h"
 #include "mega/equalize.h"
+#include "mega/fft.h"
 #include "mega/matrix.h"
 #include "mega/mega_complex.h"
 #include "mega/padded_tensor.h"
@@ -224,6 +225,46 @@ static void precode_fixture(const std::string& dir) {
   for (uint32_t i = 0; i < 2 * bs * ofdmE; ++i) EXPECT(std::fabs(out[i] - ref[i]) < 1e-5f);
 }
 
+static void fft_fixtures(const std::string& dir) {
+  std::printf("FFT fixtures\n");
+  const uint32_t bs = 8, ca = 32, num = 16, ue = 4;
+  // UplinkFFT(8, 32, 16, 8): reference output stored [subcarrier][antenna]
+  auto in = read_file<short>(dir + "/fft_uplink_in.bin");
+  auto ref = read_file<mega::Complex>(dir + "/fft_uplink_out_kb.bin");
+  Matrix d_in(2 * sizeof(short), ca, bs, Matrix::kDevice);
+  Matrix d_out(sizeof(mega::Complex), num, bs, Matrix::kDevice);
+  EXPECT(in.size() * 2 == d_in.szBytes());
+  EXPECT(hipMemcpy(d_in.ptr(), in.data(), d_in.szBytes(), hipMemcpyHostToDevice) == hipSuccess);
+  mega::UplinkFFT<short> fft(8, ca, num, bs);
+  fft(d_in, d_out);
+  std::vector<mega::Complex> out(bs * num);
+  EXPECT(hipMemcpy(out.data(), d_out.ptr(), d_out.szBytes(), hipMemcpyDeviceToHost) == hipSuccess);
+  for (uint32_t i = 0; i < num; ++i)
+    for (uint32_t j = 0; j < bs; ++j) {
+      EXPECT(std::fabs(out[j * num + i].re - ref[i * bs + j].re) < 1e-5f);
+      EXPECT(std::fabs(out[j * num + i].im - ref[i * bs + j].im) < 1e-5f);
+    }
+  // PilotFFT(8, 32, 16, 8, 4, 4) with the pilot sign of 16 subcarriers
+  auto pin = read_file<short>(dir + "/fft_pilot_in.bin");
+  auto cref = read_file<mega::Complex>(dir + "/fft_pilot_csi.bin");
+  mega::PilotSign::init(num);
+  Matrix d_pin(2 * sizeof(short), ca, bs, Matrix::kDevice);
+  Matrix d_csi(sizeof(mega::Complex), ue, bs, num / ue, Matrix::kDevice);
+  EXPECT(hipMemcpy(d_pin.ptr(), pin.data(), d_pin.szBytes(), hipMemcpyHostToDevice) == hipSuccess);
+  mega::PilotFFT<short> pfft(8, ca, num, bs, ue, ue);
+  pfft(d_pin, d_csi);
+  std::vector<mega::Complex> csi(bs * num);
+  EXPECT(hipMemcpy(csi.data(), d_csi.ptr(), d_csi.szBytes(), hipMemcpyDeviceToHost) == hipSuccess);
+  for (uint32_t i = 0; i < bs * num; ++i) {
+    EXPECT(std::fabs(csi[i].re - cref[i].re) < 1e-4f);
+    EXPECT(std::fabs(csi[i].im - cref[i].im) < 1e-4f);
+  }
+  mega::PilotSign::destroy();
+  EXPECT_THROW(pfft(d_pin, d_csi), std::runtime_error);  // no table after destroy
+  mega::UplinkFFT<short> bad(0, 48, 16, 8);              // not a power of two
+  EXPECT_THROW(bad(d_in, d_out), std::runtime_error);
+}
+
 static void beamform_fixture(const std::string& dir) {
   std::printf("Beamform fixture\n");
   const uint32_t ue = 4, bs = 8;
@@ -278,6 +319,7 @@ int main(int argc, char** argv) {
     if (mode == "gpu") {
       equalize_fixture(argv[2]);
       precode_fixture(argv[2]);
+      fft_fixtures(argv[2]);
       beamform_fixture(argv[2]);
     }
   } catch (const std::exception& e) {
