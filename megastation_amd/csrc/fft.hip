@@ -14,8 +14,9 @@
 // One workgroup of 256 threads per (antenna, symbol): samples are loaded
 // coalesced and scaled into LDS in bit-reversed order, ca/2 twiddles are
 // computed once into LDS (sincospif of exact arguments), and log2(ca)
-// radix-2 decimation-in-time stages run in place with a barrier between
-// them; the store reads the shifted bins straight from LDS. Sizes: powers of
+// radix-2 decimation-in-time stages run in place, three per barrier (radix
+// 2^3, eight elements per thread in registers); the store reads the shifted
+// bins straight from LDS. Sizes: powers of
 // two from 16 to 4096 (48 KB of LDS at 4096).
 #include "internal.h"
 
@@ -25,6 +26,44 @@ namespace {
 
 constexpr int kFftThreads = 256;
 constexpr uint32_t kFftMax = 4096;
+
+// u + w v and u - w v (complex)
+__device__ __forceinline__ float2 bfly_add(float2 u, float2 w, float2 v) {
+  return make_float2(u.x + (w.x * v.x - w.y * v.y), u.y + (w.x * v.y + w.y * v.x));
+}
+__device__ __forceinline__ float2 bfly_sub(float2 u, float2 w, float2 v) {
+  return make_float2(u.x - (w.x * v.x - w.y * v.y), u.y - (w.x * v.y + w.y * v.x));
+}
+
+// Stages st .. st+R-1 of the in-place decimation-in-time FFT over sx (input
+// in bit-reversed order), 2^R elements per thread.
+template <int R>
+__device__ __forceinline__ void fft_pass(float2* sx, const float2* tw, uint32_t N, uint32_t L,
+                                         uint32_t st) {
+  constexpr int M = 1 << R;
+  const uint32_t half = 1u << st;
+  for (uint32_t j = threadIdx.x; j < (N >> R); j += kFftThreads) {
+    const uint32_t k = j & (half - 1);
+    const uint32_t i0 = ((j >> st) << (st + R)) + k;
+    float2 v[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) v[m] = sx[i0 + m * half];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int sp = 1 << q;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (m & sp) continue;
+        const float2 w = tw[(k + (uint32_t)(m & (sp - 1)) * half) << (L - 1 - st - q)];
+        const float2 u = v[m], x = v[m + sp];
+        v[m] = bfly_add(u, w, x);
+        v[m + sp] = bfly_sub(u, w, x);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) sx[i0 + m * half] = v[m];
+  }
+}
 
 template <typename Q>
 __device__ __forceinline__ float2 load_iq(const Q* p, uint64_t n, float inv) {
@@ -49,17 +88,20 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     tw[k] = make_float2(cs, sn);
   }
   __syncthreads();
-  for (uint32_t st = 0; st < L; ++st) {
-    const uint32_t half = 1u << st;
-    for (uint32_t j = threadIdx.x; j < H; j += kFftThreads) {
-      const uint32_t k = j & (half - 1);
-      const uint32_t i0 = ((j >> st) << (st + 1)) + k, i1 = i0 + half;
-      const float2 w = tw[k << (L - 1 - st)];
-      const float2 u = sx[i0], v = sx[i1];
-      const float2 t = make_float2(w.x * v.x - w.y * v.y, w.x * v.y + w.y * v.x);
-      sx[i0] = make_float2(u.x + t.x, u.y + t.y);
-      sx[i1] = make_float2(u.x - t.x, u.y - t.y);
-    }
+  // R stages per barrier (radix 2^R, R = 3, the last pass 1 or 2): a thread
+  // takes the 2^R elements x[i + m h] (h = 2^st, i = its block's start + k,
+  // k < h) and runs stages st .. st+R-1 in registers; stage st+q pairs
+  // m and m + 2^q with W_{2^(st+q+1)}^(k + (m mod 2^q) h).
+  uint32_t st = 0;
+  for (; st + 3 <= L; st += 3) {
+    fft_pass<3>(sx, tw, N, L, st);
+    __syncthreads();
+  }
+  if (st + 2 == L) {
+    fft_pass<2>(sx, tw, N, L, st);
+    __syncthreads();
+  } else if (st + 1 == L) {
+    fft_pass<1>(sx, tw, N, L, st);
     __syncthreads();
   }
   const uint32_t shift = a.start + H;
