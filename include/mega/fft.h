@@ -1,8 +1,9 @@
-// mega::UplinkFFT / mega::PilotFFT / mega::PilotSign — the uplink FFT stage,
-// drop-in for the reference classes (src/operators/fft/fft_op.h:27-113) over
+// mega::UplinkFFT / mega::PilotFFT / mega::PilotSign / mega::DownlinkIFFT —
+// the FFT stages, drop-in for the reference classes
+// (src/operators/fft/fft_op.h:27-113) over
 // libmegadetect's C ABI: the constructor keeps the reference's arguments,
 // operator() runs the fused load + FFT + store kernel (mega_uplink_fft /
-// mega_pilot_fft) on `stream`. QuanT is short (16-bit samples) or char
+// mega_pilot_fft / mega_downlink_ifft) on `stream`. QuanT is short (16-bit samples) or char
 // (8-bit). PilotSign::init builds the pilot table on the host
 // (mega_pilot_sign) and copies it to each device, as the reference does;
 // PilotFFT reads the current device's table. Failures throw
@@ -106,6 +107,24 @@ class PilotFFT {
 
  private:
   uint32_t start_ = 0, ca_ = 0, num_ = 0, bs_ = 0, ue_ = 0, sg_ = 0;
+};
+
+template <typename QuanT = short>
+class DownlinkIFFT {
+ public:
+  DownlinkIFFT() = default;
+  DownlinkIFFT(uint32_t ofdm_start, uint32_t ofdm_ca, uint32_t ofdm_num, uint32_t bs_num)
+      : start_(ofdm_start), ca_(ofdm_ca), num_(ofdm_num), bs_(bs_num) {}
+
+  // in: [bs][ofdm_ca] Complex; out: [bs][ofdm_ca] QuanT (re, im) pairs
+  void operator()(const Matrix& in, const Matrix& out, hipStream_t stream = nullptr) const {
+    mega_fft_desc d{start_, ca_, num_, bs_, 1, fft_detail::quant_bits<QuanT>(), in.ptr(), 0,
+                    out.ptr(), 0};
+    if (mega_downlink_ifft(&d, stream) < 0) throw std::runtime_error(mega_last_error());
+  }
+
+ private:
+  uint32_t start_ = 0, ca_ = 0, num_ = 0, bs_ = 0;
 };
 
 }  // namespace mega

@@ -195,6 +195,14 @@ int mega_uplink_fft(const mega_fft_desc *desc, mega_stream_t stream);
 int mega_pilot_fft(const mega_fft_desc *desc, uint32_t ue_num, uint32_t sc_group,
                    uint32_t ue_start, const void *pilot_sign, mega_stream_t stream);
 
+/* DownlinkIFFT::operator() (fft_op.cu:168-174, ifft_store_downlink_kernel
+ * ifft_callback.cuh:21-59): in = complex64 rows [S][B][ofdm_ca] (the
+ * precode output, in_sym_stride in complex elements), out = integer
+ * (re, im) pairs [S][B][ofdm_ca] (out_sym_stride in pairs): the unnormalised
+ * inverse DFT, each part times (max + 1) / ofdm_ca, rounded to nearest even
+ * and saturated. ofdm_start and ofdm_num are not used. */
+int mega_downlink_ifft(const mega_fft_desc *desc, mega_stream_t stream);
+
 /* PilotSign::init's table (fft_op.cu:176-194), on the host: the
  * Zadoff-Chu sequence of ofdm_num (utils.h:54-75) times the pi/4 cyclic
  * shift (utils.h:77-95), each element divided by its squared magnitude;

@@ -140,6 +140,7 @@ def lib() -> ctypes.CDLL:
         "mega_uplink_fft": ([ctypes.POINTER(FftDesc), P], i32),
         "mega_pilot_fft": ([ctypes.POINTER(FftDesc), u32, u32, u32, P, P], i32),
         "mega_pilot_sign": ([u32, P], i32),
+        "mega_downlink_ifft": ([ctypes.POINTER(FftDesc), P], i32),
         "mega_precode_syms": ([ctypes.POINTER(PrecodeDesc), P], i32),
         "mega_batch_create": ([ctypes.POINTER(CellDesc), u32, ctypes.c_char_p,
                                ctypes.POINTER(P)], i32),
@@ -338,6 +339,16 @@ def pilot_fft(iq, csi, pilot, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, ue_num, 
     d = _fft_desc(iq, csi, ofdm_start, ofdm_ca, ofdm_num, bs_num, 1, 0, 0)
     _check(lib().mega_pilot_fft(ctypes.byref(d), ue_num, sc_group, ue_start, _ptr(pilot),
                                 _stream(stream)))
+
+
+def downlink_ifft(x, iq, *, ofdm_ca, bs_num, num_syms: int = 1, in_sym_stride: int = 0,
+                  out_sym_stride: int = 0, stream=None):
+    """DownlinkIFFT (mega_downlink_ifft): complex64 rows x [S][B][ca] ->
+    int16/int8 (re, im) pairs iq [S][B][ca][2]."""
+    d = _fft_desc(iq, None, 0, ofdm_ca, ofdm_ca, bs_num, num_syms, in_sym_stride,
+                  out_sym_stride)
+    d.in_, d.out = _ptr(x).value, _ptr(iq).value
+    _check(lib().mega_downlink_ifft(ctypes.byref(d), _stream(stream)))
 
 
 def pilot_sign(ofdm_num: int):

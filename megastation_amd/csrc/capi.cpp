@@ -308,11 +308,12 @@ int mega_equalize(uint64_t row, uint64_t col, uint64_t batch_count,
   return mega_equalize_slots(&d, stream);
 }
 
-static int fft_checked(const mega_fft_desc* d, bool pilot, uint32_t ue_num, uint32_t sc_group,
+static int fft_checked(const mega_fft_desc* d, int mode, uint32_t ue_num, uint32_t sc_group,
                        uint32_t ue_start, const void* pilot_sign, mega_stream_t stream,
                        const char* who) {
+  const bool pilot = mode == 1;
   if (!d) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": desc is NULL");
-  if (d->bs_num == 0 || d->num_syms == 0 || d->ofdm_num == 0) return MEGA_OK;
+  if (d->bs_num == 0 || d->num_syms == 0 || (mode != 2 && d->ofdm_num == 0)) return MEGA_OK;
   if (!d->in || !d->out) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
   if (pilot && !pilot_sign)
     return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL pilot sign table");
@@ -322,6 +323,7 @@ static int fft_checked(const mega_fft_desc* d, bool pilot, uint32_t ue_num, uint
   std::memset(&a, 0, sizeof a);
   a.in = d->in;
   a.out = static_cast<float2*>(d->out);
+  a.qout = d->out;
   a.pilot = static_cast<const float2*>(pilot_sign);
   a.B = d->bs_num;
   a.S = d->num_syms;
@@ -334,22 +336,27 @@ static int fft_checked(const mega_fft_desc* d, bool pilot, uint32_t ue_num, uint
   a.ue_start = ue_start;
   a.in_sym = d->in_sym_stride ? d->in_sym_stride : (uint64_t)d->bs_num * d->ofdm_ca;
   a.out_sym = d->out_sym_stride ? d->out_sym_stride
-              : pilot ? (uint64_t)((d->ofdm_num + sc_group - 1) / sc_group) * d->bs_num * ue_num
-                      : (uint64_t)d->bs_num * d->ofdm_num;
+              : pilot     ? (uint64_t)((d->ofdm_num + sc_group - 1) / sc_group) * d->bs_num * ue_num
+              : mode == 2 ? (uint64_t)d->bs_num * d->ofdm_ca
+                          : (uint64_t)d->bs_num * d->ofdm_num;
   const char* why = nullptr;
-  hipError_t e = mega::detail::launch_fft(a, pilot, static_cast<hipStream_t>(stream), &why);
+  hipError_t e = mega::detail::launch_fft(a, mode, static_cast<hipStream_t>(stream), &why);
   if (e == hipErrorInvalidValue && why) return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": " + why);
   if (e != hipSuccess) return hip_fail(e, who, why);
   return MEGA_OK;
 }
 
 int mega_uplink_fft(const mega_fft_desc* d, mega_stream_t stream) {
-  return fft_checked(d, false, 0, 0, 0, nullptr, stream, "mega_uplink_fft");
+  return fft_checked(d, 0, 0, 0, 0, nullptr, stream, "mega_uplink_fft");
+}
+
+int mega_downlink_ifft(const mega_fft_desc* d, mega_stream_t stream) {
+  return fft_checked(d, 2, 0, 0, 0, nullptr, stream, "mega_downlink_ifft");
 }
 
 int mega_pilot_fft(const mega_fft_desc* d, uint32_t ue_num, uint32_t sc_group, uint32_t ue_start,
                    const void* pilot_sign, mega_stream_t stream) {
-  return fft_checked(d, true, ue_num, sc_group, ue_start, pilot_sign, stream, "mega_pilot_fft");
+  return fft_checked(d, 1, ue_num, sc_group, ue_start, pilot_sign, stream, "mega_pilot_fft");
 }
 
 // Largest prime <= n among the primes up to 2039, else 2039: the length rule

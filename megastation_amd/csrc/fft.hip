@@ -10,6 +10,10 @@
 //   pilot:  the CSI element of user ue_start + i % sg in group i / sg,
 //           conj(X[col(i)]) * pilot[i], into [G][B][U] (A = H^H, the layout
 //           the ZF kernels read); users >= U are skipped.
+// The downlink IFFT (DownlinkIFFT, fft_op.cu:168-174, ifft_callback.cuh:21-59)
+// runs the same kernel the other way: complex rows [S][B][ca] (the precode
+// output), the unnormalised inverse DFT, each part scaled by (max + 1) / ca,
+// rounded to nearest even and saturated into integer (re, im) pairs.
 //
 // One workgroup of 256 threads per (antenna, symbol): samples are loaded
 // coalesced and scaled into LDS in bit-reversed order, ca/2 twiddles are
@@ -70,21 +74,38 @@ __device__ __forceinline__ float2 load_iq(const Q* p, uint64_t n, float inv) {
   return make_float2((float)p[2 * n] * inv, (float)p[2 * n + 1] * inv);
 }
 
-template <typename Q, bool PILOT>
+enum FftMode { kUplink = 0, kPilot = 1, kIfft = 2 };
+
+template <typename Q>
+__device__ __forceinline__ Q quantize(float v) {
+  constexpr float hi = sizeof(Q) == 2 ? 32767.f : 127.f;
+  constexpr float lo = sizeof(Q) == 2 ? -32768.f : -128.f;
+  if (v > hi) return (Q)hi;
+  if (v < lo) return (Q)lo;
+  return (Q)(int)__builtin_rintf(v);
+}
+
+template <typename Q, int MODE>
 __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   __shared__ float2 sx[kFftMax];
   __shared__ float2 tw[kFftMax / 2];
   const uint32_t b = blockIdx.x, s = blockIdx.y;
   const uint32_t N = a.ca, L = a.log2ca, H = N / 2;
-  const Q* in = static_cast<const Q*>(a.in) + 2 * ((uint64_t)s * a.in_sym + (uint64_t)b * N);
-  const float inv = 1.0f / a.qscale;
-  for (uint32_t n = threadIdx.x; n < N; n += kFftThreads) {
-    const uint32_t r = __builtin_bitreverse32(n) >> (32 - L);
-    sx[r] = load_iq(in, n, inv);
+  const uint64_t row = (uint64_t)s * a.in_sym + (uint64_t)b * N;
+  if constexpr (MODE == kIfft) {
+    const float2* in = static_cast<const float2*>(a.in) + row;
+    for (uint32_t n = threadIdx.x; n < N; n += kFftThreads)
+      sx[__builtin_bitreverse32(n) >> (32 - L)] = in[n];
+  } else {
+    const Q* in = static_cast<const Q*>(a.in) + 2 * row;
+    const float inv = 1.0f / a.qscale;
+    for (uint32_t n = threadIdx.x; n < N; n += kFftThreads)
+      sx[__builtin_bitreverse32(n) >> (32 - L)] = load_iq(in, n, inv);
   }
+  const float dir = MODE == kIfft ? 2.0f : -2.0f;  // exp(+-2 pi i k / N)
   for (uint32_t k = threadIdx.x; k < H; k += kFftThreads) {
     float sn, cs;
-    sincospif(-2.0f * (float)k / (float)N, &sn, &cs);  // exp(-2 pi i k / N)
+    sincospif(dir * (float)k / (float)N, &sn, &cs);
     tw[k] = make_float2(cs, sn);
   }
   __syncthreads();
@@ -105,7 +126,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     __syncthreads();
   }
   const uint32_t shift = a.start + H;
-  if constexpr (!PILOT) {
+  if constexpr (MODE == kIfft) {
+    Q* o = static_cast<Q*>(a.qout) + 2 * ((uint64_t)s * a.out_sym + (uint64_t)b * N);
+    const float scale = a.qscale / (float)N;
+    for (uint32_t n = threadIdx.x; n < N; n += kFftThreads) {
+      const float2 v = sx[n];
+      o[2 * n] = quantize<Q>(v.x * scale);
+      o[2 * n + 1] = quantize<Q>(v.y * scale);
+    }
+  } else if constexpr (MODE == kUplink) {
     float2* y = a.out + (uint64_t)s * a.out_sym + (uint64_t)b * a.num;
     for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) y[i] = sx[(i + shift) % N];
   } else {
@@ -120,25 +149,26 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   }
 }
 
-template <bool PILOT>
+template <int MODE>
 hipError_t launch_q(const FftArgs& a, hipStream_t st) {
   const dim3 grid(a.B, a.S), block(kFftThreads);
   if (a.qbits == 16)
-    hipLaunchKernelGGL((fft_kernel<int16_t, PILOT>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((fft_kernel<int16_t, MODE>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((fft_kernel<int8_t, PILOT>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((fft_kernel<int8_t, MODE>), grid, block, 0, st, a);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_fft(const FftArgs& a0, bool pilot, hipStream_t st, const char** why) {
+hipError_t launch_fft(const FftArgs& a0, int mode, hipStream_t st, const char** why) {
   FftArgs a = a0;
-  if (a.B == 0 || a.S == 0 || a.num == 0) return hipSuccess;
+  if (a.B == 0 || a.S == 0 || (mode != kIfft && a.num == 0)) return hipSuccess;
   if (a.ca < 16 || a.ca > kFftMax || (a.ca & (a.ca - 1)) != 0) {
     *why = "ofdm_ca must be a power of two from 16 to 4096";
     return hipErrorInvalidValue;
   }
+  if (mode == kIfft) a.num = a.ca;  // every bin
   if (a.num > a.ca || a.start >= a.ca) {
     *why = "ofdm_num and ofdm_start must be below ofdm_ca";
     return hipErrorInvalidValue;
@@ -151,10 +181,11 @@ hipError_t launch_fft(const FftArgs& a0, bool pilot, hipStream_t st, const char*
     *why = "grid too large";
     return hipErrorInvalidValue;
   }
-  if (pilot && (a.sg == 0 || a.U == 0)) return hipSuccess;
+  if (mode == kPilot && (a.sg == 0 || a.U == 0)) return hipSuccess;
   a.log2ca = 31 - __builtin_clz(a.ca);
   a.qscale = a.qbits == 16 ? 32768.0f : 128.0f;
-  return pilot ? launch_q<true>(a, st) : launch_q<false>(a, st);
+  if (mode == kIfft) return launch_q<kIfft>(a, st);
+  return mode == kPilot ? launch_q<kPilot>(a, st) : launch_q<kUplink>(a, st);
 }
 
 }  // namespace detail

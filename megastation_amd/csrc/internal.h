@@ -90,6 +90,7 @@ hipError_t launch_precode(const PrecodeArgs& a, hipStream_t stream, const char**
 struct FftArgs {
   const void* in;
   float2* out;
+  void* qout;           // downlink IFFT only: integer pairs
   const float2* pilot;  // pilot only: [num]
   uint32_t B, S, ca, num, start, qbits;
   uint32_t U, sg, ue_start;  // pilot only
@@ -97,7 +98,8 @@ struct FftArgs {
   uint32_t log2ca;  // set by the launcher
   float qscale;     // set by the launcher
 };
-hipError_t launch_fft(const FftArgs& a, bool pilot, hipStream_t stream, const char** why);
+// mode: 0 uplink, 1 pilot, 2 downlink IFFT
+hipError_t launch_fft(const FftArgs& a, int mode, hipStream_t stream, const char** why);
 
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,

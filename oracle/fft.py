@@ -11,7 +11,8 @@ The transform itself is numpy's FFT in complex128 (the textbook DFT
 X[k] = sum_n x[n] exp(-2 pi i k n / N)); everything around it in FP32 as
 the reference computes it. Pinned by the reference's own fixtures
 (tests/test_oracle_fft.py): test/fft/fft_out.data, complex_out.data,
-zachu.data, zachu_shift.data.
+zachu.data, zachu_shift.data, ifft_out.data (DownlinkIFFT, fft_op.cu:168-174,
+ifft_callback.cuh:21-59).
 """
 from __future__ import annotations
 
@@ -135,3 +136,21 @@ def pilot_fft(iq: np.ndarray, ofdm_start: int, ca: int, num: int, bs: int, ue: i
             continue
         o[i // sg, :, u] = re[:, i] + 1j * im[:, i]
     return o
+
+
+def downlink_ifft(x: np.ndarray, ca: int, bs: int, qdtype=np.int16) -> np.ndarray:
+    """DownlinkIFFT (fft_op.cu:168-174, ifft_store_downlink_kernel
+    ifft_callback.cuh:21-59): x [bs][ca] complex64 -> (re, im) integer pairs
+    [bs][ca][2]: the unnormalised inverse DFT (cuFFT INVERSE), each part
+    scaled by (max + 1) / ca in float, rounded to nearest even, saturated."""
+    X = np.fft.ifft(x.reshape(bs, ca).astype(np.complex128), axis=1) * ca
+    X = X.astype(np.complex64)
+    info = np.iinfo(qdtype)
+    scale = np.float32(np.float32(info.max + 1.0) / np.float32(ca))
+    out = np.empty((bs, ca, 2), qdtype)
+    for c, part in enumerate((X.real, X.imag)):
+        v = part.astype(np.float32) * scale
+        q = np.rint(v)
+        q = np.where(v > info.max, info.max, np.where(v < info.min, info.min, q))
+        out[..., c] = q.astype(qdtype)
+    return out

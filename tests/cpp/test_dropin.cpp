@@ -263,6 +263,17 @@ static void fft_fixtures(const std::string& dir) {
   EXPECT_THROW(pfft(d_pin, d_csi), std::runtime_error);  // no table after destroy
   mega::UplinkFFT<short> bad(0, 48, 16, 8);              // not a power of two
   EXPECT_THROW(bad(d_in, d_out), std::runtime_error);
+  // DownlinkIFFT(8, 32, 16, 8): int16 output compared exactly (:162-206)
+  auto iin = read_file<mega::Complex>(dir + "/ifft_in.bin");
+  auto iref = read_file<short>(dir + "/ifft_out.bin");
+  Matrix d_iin(sizeof(mega::Complex), ca, bs, Matrix::kDevice);
+  Matrix d_iout(2 * sizeof(short), ca, bs, Matrix::kDevice);
+  EXPECT(hipMemcpy(d_iin.ptr(), iin.data(), d_iin.szBytes(), hipMemcpyHostToDevice) == hipSuccess);
+  mega::DownlinkIFFT<short> ifft(8, ca, num, bs);
+  ifft(d_iin, d_iout);
+  std::vector<short> iout(2 * bs * ca);
+  EXPECT(hipMemcpy(iout.data(), d_iout.ptr(), d_iout.szBytes(), hipMemcpyDeviceToHost) == hipSuccess);
+  for (uint32_t i = 0; i < 2 * bs * ca; ++i) EXPECT(iout[i] == iref[i]);
 }
 
 static void beamform_fixture(const std::string& dir) {

@@ -28,6 +28,8 @@ Sources (binary fixtures next to the reference tests):
   test/fft/zachu.data       Zadoff-Chu sequence of length 32, first 31 values
                             (complex64, 1e-5; :10-30)
   test/fft/zachu_shift.data Zadoff-Chu of 16 times the pi/4 cyclic shift (:32-54)
+  test/fft/ifft_in.data     downlink IFFT input, [bs=8][32] complex64 (:162-206)
+  test/fft/ifft_out.data    expected int16 (re, im) pairs, [bs=8][32] (exact)
 
 The arrays are stored unchanged (raw little-endian values); this script only
 re-packages them so the tests run where the reference tree is absent.
@@ -64,6 +66,8 @@ def main():
         "fft_pilot_csi": rd("test/fft/complex_out.data", np.complex64),
         "zadoff_chu_32": rd("test/fft/zachu.data", np.complex64),
         "zadoff_chu_16_shift": rd("test/fft/zachu_shift.data", np.complex64),
+        "ifft_in": rd("test/fft/ifft_in.data", np.complex64),
+        "ifft_out": rd("test/fft/ifft_out.data", np.int16),
     }
     assert arrays["sgemv_csi"].size == 4 * 8
     assert arrays["sgemv_bsvec_kb"].size == 16 * 8
@@ -78,6 +82,8 @@ def main():
     assert arrays["fft_pilot_csi"].size == 4 * 8 * 4
     assert arrays["zadoff_chu_32"].size == 31
     assert arrays["zadoff_chu_16_shift"].size == 16
+    assert arrays["ifft_in"].size == 8 * 32
+    assert arrays["ifft_out"].size == 2 * 8 * 32
     np.savez(OUT, **arrays)
     print("wrote", OUT)
 

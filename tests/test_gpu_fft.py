@@ -95,3 +95,42 @@ def test_pilot_fft_matches_oracle(B, ca, num, start, U, sg, ue_start):
     assert np.array_equal(got[untouched], exp[untouched])
     scale = float(np.abs(exp[~untouched]).max())
     np.testing.assert_allclose(got, exp, rtol=0, atol=1e-5 * max(1.0, scale))
+
+
+def test_downlink_ifft_reference_fixture(golden):
+    print("case: downlink IFFT fixture 8 antennas, 32 points", flush=True)
+    t = torch()
+    iq = t.zeros(8 * 32 * 2, dtype=t.int16, device="cuda:0")
+    m.downlink_ifft(dev_c64(golden["ifft_in"]), iq, ofdm_ca=32, bs_num=8)
+    t.cuda.synchronize()
+    assert np.array_equal(iq.cpu().numpy(), golden["ifft_out"])
+
+
+@pytest.mark.parametrize("B,ca,S,dtype,amp", [
+    (64, 2048, 2, np.int16, 3e-3),   # typical level
+    (16, 4096, 1, np.int16, 0.5),    # overdriven: saturation
+    (8, 512, 2, np.int8, 5e-3),
+])
+def test_downlink_ifft_matches_oracle(B, ca, S, dtype, amp):
+    """FP32 radix-2 against numpy's complex128 inverse: integers equal except
+    where the scaled value lies within 1e-3 of a rounding boundary, there at
+    most one apart; saturation exact."""
+    print(f"case: downlink IFFT B={B} ca={ca} S={S} {dtype.__name__} amp={amp}", flush=True)
+    t = torch()
+    rng = np.random.default_rng(ca + S)
+    x = ((rng.standard_normal((S, B, ca)) + 1j * rng.standard_normal((S, B, ca))) * amp)
+    x = x.astype(np.complex64)
+    iq = t.zeros(S * B * ca * 2, dtype=t.int16 if dtype == np.int16 else t.int8, device="cuda:0")
+    m.downlink_ifft(dev_c64(x.reshape(-1)), iq, ofdm_ca=ca, bs_num=B, num_syms=S)
+    t.cuda.synchronize()
+    got = iq.cpu().numpy().reshape(S, B, ca, 2).astype(np.int32)
+    info = np.iinfo(dtype)
+    for s in range(S):
+        exp = offt.downlink_ifft(x[s], ca, B, dtype).astype(np.int32)
+        Xs = np.fft.ifft(x[s].astype(np.complex128), axis=1) * ca * ((info.max + 1.0) / ca)
+        v = np.stack([Xs.real, Xs.imag], -1)
+        near = np.abs(np.abs(v - np.floor(v)) - 0.5) < 1e-3
+        d = np.abs(got[s] - exp)
+        assert (d <= 1).all() and (d[~near] == 0).all()
+        sat = (v > info.max) | (v < info.min)
+        assert np.array_equal(got[s][sat], exp[sat])

@@ -69,3 +69,21 @@ def test_pilot_store_recovers_channel():
                                    atol=2e-3)
     # users past ue are never written (out stays zero there)
     assert not csi[:, :, 0].any()
+
+
+def test_downlink_ifft_fixture(golden):
+    """test_fft.cc:162-206: DownlinkIFFT(8, 32, 16, 8), int16 output compared
+    exactly."""
+    out = offt.downlink_ifft(golden["ifft_in"], 32, 8)
+    assert np.array_equal(out.reshape(-1), golden["ifft_out"])
+
+
+def test_downlink_ifft_inverts_uplink():
+    """Round trip: samples through the uplink transform (all bins) and back
+    through the downlink one give the samples again (quantisation aside)."""
+    rng = np.random.default_rng(2)
+    bs, ca = 3, 256
+    iq = rng.integers(-6000, 6000, size=(bs, ca, 2)).astype(np.int16)
+    X = np.fft.fft(offt._load(iq, bs, ca), axis=1).astype(np.complex64)
+    back = offt.downlink_ifft(X, ca, bs)
+    assert np.abs(back.astype(np.int32) - iq.astype(np.int32)).max() <= 1
