@@ -669,8 +669,8 @@ def main_fft(args):
     """Uplink FFT stage of a 256x32 cell (mega_pilot_fft, mega_uplink_fft):
     4096-point symbols of 16-bit (re, im) samples, 3276 subcarriers from bin
     offset 410; one step = one slot: three pilot symbols (32 users in groups
-    of 12, users 0-11, 12-23, 24-31) into the CSI [273][256][32], and 13
-    data symbols into y [13][256][3276] in one launch. Four rotating buffer
+    of 12, users 0-11, 12-23, 24-31) into the CSI [273][256][32] in one
+    launch, and 13 data symbols into y [13][256][3276] in one launch. Four rotating buffer
     sets. The CPU baseline is numpy's FFT (the oracle's) on one symbol."""
     import torch
     import megastation_amd as m
@@ -695,9 +695,7 @@ def main_fft(args):
         j %= NR
         if evs:
             evs[0].record(stream)
-        for p in range(P):
-            m.pilot_fft(PL[j][p * B * ca * 2:], C[j], psign, ue_num=U, sc_group=sg,
-                        ue_start=p * sg, **kw)
+        m.pilot_fft(PL[j], C[j], psign, ue_num=U, sc_group=sg, num_syms=P, **kw)
         if evs:
             evs[1].record(stream)
         m.uplink_fft(D[j], Y[j], num_syms=S, **kw)

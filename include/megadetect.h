@@ -185,8 +185,12 @@ int mega_uplink_fft(const mega_fft_desc *desc, mega_stream_t stream);
 
 /* PilotFFT::operator()(in, out, ue_start) (fft_op.cu:152-172,
  * fft_store_pilot_kernel fft_callback.cuh:76-108): pilot subcarrier i of
- * antenna b gives the CSI element of user ue_start + i % sc_group in group
- * i / sc_group, conj(X[col(i)]) * pilot_sign[i] computed as
+ * antenna b in pilot symbol s gives the CSI element of user
+ * ue_start + s * sc_group + i % sc_group in group i / sc_group (s = 0 for
+ * one symbol, the reference's case; num_syms > 1 handles the pilot symbols
+ * of a slot in one launch, each carrying the next sc_group users, all into
+ * the same CSI unless out_sym_stride is set), conj(X[col(i)]) * pilot_sign[i]
+ * computed as
  * {e.re p.re + e.im p.im, e.re p.im - e.im p.re}, written to out =
  * [G][B][ue_num] complex64 (G = ceil(ofdm_num / sc_group); A = H^H, the
  * layout mega_zf / mega_beamform_uplink read); users >= ue_num are skipped

@@ -333,10 +333,11 @@ def uplink_fft(iq, y, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms: int = 
 
 
 def pilot_fft(iq, csi, pilot, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, ue_num, sc_group,
-              ue_start: int = 0, stream=None):
-    """PilotFFT (mega_pilot_fft): one pilot symbol iq [B][ca][2] -> CSI
-    [G][B][U] complex64 (A = H^H); pilot: device complex64 [ofdm_num]."""
-    d = _fft_desc(iq, csi, ofdm_start, ofdm_ca, ofdm_num, bs_num, 1, 0, 0)
+              ue_start: int = 0, num_syms: int = 1, stream=None):
+    """PilotFFT (mega_pilot_fft): pilot symbols iq [S][B][ca][2] -> CSI
+    [G][B][U] complex64 (A = H^H), symbol s carrying users ue_start +
+    s*sc_group + (i % sc_group); pilot: device complex64 [ofdm_num]."""
+    d = _fft_desc(iq, csi, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms, 0, 0)
     _check(lib().mega_pilot_fft(ctypes.byref(d), ue_num, sc_group, ue_start, _ptr(pilot),
                                 _stream(stream)))
 

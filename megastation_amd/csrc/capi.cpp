@@ -335,8 +335,10 @@ static int fft_checked(const mega_fft_desc* d, int mode, uint32_t ue_num, uint32
   a.sg = sc_group;
   a.ue_start = ue_start;
   a.in_sym = d->in_sym_stride ? d->in_sym_stride : (uint64_t)d->bs_num * d->ofdm_ca;
+  // pilot symbols of one launch fill one CSI buffer (their users differ)
+  // unless a stride is given
   a.out_sym = d->out_sym_stride ? d->out_sym_stride
-              : pilot     ? (uint64_t)((d->ofdm_num + sc_group - 1) / sc_group) * d->bs_num * ue_num
+              : pilot     ? 0
               : mode == 2 ? (uint64_t)d->bs_num * d->ofdm_ca
                           : (uint64_t)d->bs_num * d->ofdm_num;
   const char* why = nullptr;

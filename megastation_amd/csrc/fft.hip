@@ -7,9 +7,10 @@
 // then either
 //   uplink: y[s][b][i] = X[(i + start + ca/2) % ca], i < num  (the [B][N]
 //           layout the equalize reads), or
-//   pilot:  the CSI element of user ue_start + i % sg in group i / sg,
+//   pilot:  the CSI element of user ue_start + s*sg + i % sg in group i / sg,
 //           conj(X[col(i)]) * pilot[i], into [G][B][U] (A = H^H, the layout
-//           the ZF kernels read); users >= U are skipped.
+//           the ZF kernels read); users >= U are skipped. Pilot symbol s of a
+//           launch carries the s-th set of sg users.
 // The downlink IFFT (DownlinkIFFT, fft_op.cu:168-174, ifft_callback.cuh:21-59)
 // runs the same kernel the other way: complex rows [S][B][ca] (the precode
 // output), the unnormalised inverse DFT, each part scaled by (max + 1) / ca,
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) y[i] = sx[(i + shift) % N];
   } else {
     for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) {
-      const uint32_t u = a.ue_start + i % a.sg;
+      const uint32_t u = a.ue_start + s * a.sg + i % a.sg;
       if (u >= a.U) continue;
       const float2 e = sx[(i + shift) % N], p = a.pilot[i];
       const uint64_t o = (uint64_t)(i / a.sg) * a.B * a.U + (uint64_t)b * a.U + u;

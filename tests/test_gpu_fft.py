@@ -134,3 +134,28 @@ def test_downlink_ifft_matches_oracle(B, ca, S, dtype, amp):
         assert (d <= 1).all() and (d[~near] == 0).all()
         sat = (v > info.max) | (v < info.min)
         assert np.array_equal(got[s][sat], exp[sat])
+
+
+def test_pilot_fft_symbols_of_a_slot_in_one_launch():
+    """num_syms = 3: pilot symbol s carries users s*sg .. s*sg+sg-1 into one
+    CSI; equal bit for bit to three one-symbol calls with ue_start = s*sg
+    (the reference's calls), users past U skipped."""
+    print("case: three pilot symbols in one launch", flush=True)
+    t = torch()
+    rng = np.random.default_rng(31)
+    B, ca, num, start, U, sg, P = 32, 1024, 600, 212, 30, 12, 3
+    iq = t.from_numpy(rng.integers(-9000, 9000, size=(P, B, ca, 2)).astype(np.int16)).to("cuda:0")
+    p = dev_c64(m.pilot_sign(num))
+    G = (num + sg - 1) // sg
+    one = t.from_numpy(np.full(G * B * U, SENT, np.complex64)).to("cuda:0")
+    sep = one.clone()
+    kw = dict(ofdm_start=start, ofdm_ca=ca, ofdm_num=num, bs_num=B, ue_num=U, sc_group=sg)
+    m.pilot_fft(iq, one, p, num_syms=P, **kw)
+    for s in range(P):
+        m.pilot_fft(iq[s], sep, p, ue_start=s * sg, **kw)
+    t.cuda.synchronize()
+    a, b = host_c64(one), host_c64(sep)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    # users 30..35 of the third symbol are past U: exactly users 0..29 written
+    written = (a.reshape(G, B, U) != SENT).any(axis=(0, 1))
+    assert written.all()
