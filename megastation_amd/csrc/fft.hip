@@ -22,7 +22,7 @@
 // radix-2 decimation-in-time stages run in place, three per barrier (radix
 // 2^3, eight elements per thread in registers); the store reads the shifted
 // bins straight from LDS. Sizes: powers of
-// two from 16 to 4096 (48 KB of LDS at 4096).
+// two from 16 to 4096 (50 KB of LDS at 4096).
 #include "internal.h"
 
 namespace mega {
@@ -31,6 +31,11 @@ namespace {
 
 constexpr int kFftThreads = 256;
 constexpr uint32_t kFftMax = 4096;
+
+// LDS slot of element i: one pad element every 32, so the power-of-two
+// strides of the early passes (64 and 512 bytes across lanes) spread over
+// the banks.
+__device__ __forceinline__ uint32_t pidx(uint32_t i) { return i + (i >> 5); }
 
 // u + w v and u - w v (complex)
 __device__ __forceinline__ float2 bfly_add(float2 u, float2 w, float2 v) {
@@ -52,7 +57,7 @@ __device__ __forceinline__ void fft_pass(float2* sx, const float2* tw, uint32_t 
     const uint32_t i0 = ((j >> st) << (st + R)) + k;
     float2 v[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) v[m] = sx[i0 + m * half];
+    for (int m = 0; m < M; ++m) v[m] = sx[pidx(i0 + m * half)];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int sp = 1 << q;
@@ -66,7 +71,7 @@ __device__ __forceinline__ void fft_pass(float2* sx, const float2* tw, uint32_t 
       }
     }
 #pragma unroll
-    for (int m = 0; m < M; ++m) sx[i0 + m * half] = v[m];
+    for (int m = 0; m < M; ++m) sx[pidx(i0 + m * half)] = v[m];
   }
 }
 
@@ -88,7 +93,7 @@ __device__ __forceinline__ Q quantize(float v) {
 
 template <typename Q, int MODE>
 __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
-  __shared__ float2 sx[kFftMax];
+  __shared__ float2 sx[kFftMax + kFftMax / 32];
   __shared__ float2 tw[kFftMax / 2];
   const uint32_t b = blockIdx.x, s = blockIdx.y;
   const uint32_t N = a.ca, L = a.log2ca, H = N / 2;
@@ -96,12 +101,12 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   if constexpr (MODE == kIfft) {
     const float2* in = static_cast<const float2*>(a.in) + row;
     for (uint32_t n = threadIdx.x; n < N; n += kFftThreads)
-      sx[__builtin_bitreverse32(n) >> (32 - L)] = in[n];
+      sx[pidx(__builtin_bitreverse32(n) >> (32 - L))] = in[n];
   } else {
     const Q* in = static_cast<const Q*>(a.in) + 2 * row;
     const float inv = 1.0f / a.qscale;
     for (uint32_t n = threadIdx.x; n < N; n += kFftThreads)
-      sx[__builtin_bitreverse32(n) >> (32 - L)] = load_iq(in, n, inv);
+      sx[pidx(__builtin_bitreverse32(n) >> (32 - L))] = load_iq(in, n, inv);
   }
   const float dir = MODE == kIfft ? 2.0f : -2.0f;  // exp(+-2 pi i k / N)
   for (uint32_t k = threadIdx.x; k < H; k += kFftThreads) {
@@ -131,18 +136,18 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     Q* o = static_cast<Q*>(a.qout) + 2 * ((uint64_t)s * a.out_sym + (uint64_t)b * N);
     const float scale = a.qscale / (float)N;
     for (uint32_t n = threadIdx.x; n < N; n += kFftThreads) {
-      const float2 v = sx[n];
+      const float2 v = sx[pidx(n)];
       o[2 * n] = quantize<Q>(v.x * scale);
       o[2 * n + 1] = quantize<Q>(v.y * scale);
     }
   } else if constexpr (MODE == kUplink) {
     float2* y = a.out + (uint64_t)s * a.out_sym + (uint64_t)b * a.num;
-    for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) y[i] = sx[(i + shift) % N];
+    for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) y[i] = sx[pidx((i + shift) % N)];
   } else {
     for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) {
       const uint32_t u = a.ue_start + s * a.sg + i % a.sg;
       if (u >= a.U) continue;
-      const float2 e = sx[(i + shift) % N], p = a.pilot[i];
+      const float2 e = sx[pidx((i + shift) % N)], p = a.pilot[i];
       const uint64_t o = (uint64_t)(i / a.sg) * a.B * a.U + (uint64_t)b * a.U + u;
       a.out[(uint64_t)s * a.out_sym + o] =
           make_float2(e.x * p.x + e.y * p.y, e.x * p.y - e.y * p.x);
