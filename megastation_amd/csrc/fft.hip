@@ -120,11 +120,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   // k < h) and runs stages st .. st+R-1 in registers; stage st+q pairs
   // m and m + 2^q with W_{2^(st+q+1)}^(k + (m mod 2^q) h).
   uint32_t st = 0;
-  for (; st + 3 <= L; st += 3) {
-    fft_pass<3>(sx, tw, N, L, st);
+  for (; st + 4 <= L; st += 4) {
+    fft_pass<4>(sx, tw, N, L, st);
     __syncthreads();
   }
-  if (st + 2 == L) {
+  if (st + 3 == L) {
+    fft_pass<3>(sx, tw, N, L, st);
+    __syncthreads();
+  } else if (st + 2 == L) {
     fft_pass<2>(sx, tw, N, L, st);
     __syncthreads();
   } else if (st + 1 == L) {
