@@ -19,9 +19,9 @@
 // One workgroup of 256 threads per (antenna, symbol): samples are loaded
 // coalesced and scaled into LDS in bit-reversed order, ca/2 twiddles are
 // computed once into LDS (sincospif of exact arguments), and log2(ca)
-// radix-2 decimation-in-time stages run in place, three per barrier (radix
-// 2^3, eight elements per thread in registers); the store reads the shifted
-// bins straight from LDS. Sizes: powers of
+// radix-2 decimation-in-time stages run in place, four per barrier (radix
+// 2^4, sixteen elements per thread in registers); the store reads the
+// shifted bins straight from LDS. Sizes: powers of
 // two from 16 to 4096 (50 KB of LDS at 4096).
 #include "internal.h"
 
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     tw[k] = make_float2(cs, sn);
   }
   __syncthreads();
-  // R stages per barrier (radix 2^R, R = 3, the last pass 1 or 2): a thread
+  // R stages per barrier (radix 2^R, R = 4, the last pass 1 to 3): a thread
   // takes the 2^R elements x[i + m h] (h = 2^st, i = its block's start + k,
   // k < h) and runs stages st .. st+R-1 in registers; stage st+q pairs
   // m and m + 2^q with W_{2^(st+q+1)}^(k + (m mod 2^q) h).
