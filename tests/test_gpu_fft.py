@@ -159,3 +159,61 @@ def test_pilot_fft_symbols_of_a_slot_in_one_launch():
     # users 30..35 of the third symbol are past U: exactly users 0..29 written
     written = (a.reshape(G, B, U) != SENT).any(axis=(0, 1))
     assert written.all()
+
+
+def test_fft_stage_feeds_detection():
+    """The stages compose: a slot built in the frequency domain (i.i.d.
+    channel, 16-QAM at 40 dB), taken to int16 time-domain samples, through
+    mega_pilot_fft (CSI, A = H^H) -> mega_zf -> mega_uplink_fft (y) ->
+    mega_equalize_slots; the receiver's gain cancels in ZF, so x_hat must
+    give back the transmitted symbols (every nearest-point decision right,
+    EVM of the chain within 0.5 dB of equalizing the frequency-domain slot
+    directly)."""
+    print("case: pilot FFT -> ZF -> uplink FFT -> equalize", flush=True)
+    t = torch()
+    from megastation_amd import synth
+    B, U, ca, N, start, sg, S, Q = 32, 8, 256, 192, 32, 8, 2, 4
+    G = N // sg
+    rng = np.random.Generator(np.random.Philox(123))
+    sl = synth.make_slot(rng, U, B, N, sg, S, Q, 40.0, "iid")
+    p = m.pilot_sign(N)
+    cols = (np.arange(N) + start + ca // 2) % ca
+    Xp = np.zeros((B, ca), np.complex128)
+    for i in range(N):  # subcarrier i carries user i % sg; transmitted 1/conj(p)
+        Xp[:, cols[i]] = sl.H[i // sg, :, i % sg] / np.conj(p[i])
+    Xd = np.zeros((S, B, ca), np.complex128)
+    Xd[:, :, cols] = sl.y
+    xt = np.fft.ifft(np.concatenate([Xp[None], Xd]), axis=2)
+    gain = 0.4 / np.abs(np.stack([xt.real, xt.imag])).max()  # one receiver gain
+    iq = np.round(np.stack([xt.real, xt.imag], -1) * 32768 * gain).astype(np.int16)
+    dev = "cuda:0"
+    kw = dict(ofdm_start=start, ofdm_ca=ca, ofdm_num=N, bs_num=B)
+    csi = t.zeros(G * B * U, dtype=t.complex64, device=dev)
+    m.pilot_fft(t.from_numpy(iq[0].copy()).to(dev), csi, dev_c64(p), ue_num=U, sc_group=sg, **kw)
+    w = t.empty_like(csi)
+    m.zf(csi, w, B, U, G)
+    y = t.empty(S * B * N, dtype=t.complex64, device=dev)
+    m.uplink_fft(t.from_numpy(iq[1:].copy()).to(dev), y, num_syms=S, **kw)
+    lay = synth.standalone_layout(N, Q)
+    eq = dict(num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, num_slots=1, mod_order=Q,
+              ue_stride=lay.ue_stride, cb_size=lay.cb_size, cb_stride=lay.cb_stride,
+              zeros=lay.zeros, valid_bits=lay.valid_bits, csi_slot_stride=G * B * U,
+              y_sym_stride=B * N, y_slot_stride=S * B * N, llr_sym_stride=U * lay.ue_stride,
+              llr_slot_stride=S * U * lay.ue_stride)
+    llr = t.empty(S * U * lay.ue_stride, dtype=t.int16, device=dev)
+    xh = t.zeros(S * U * N, dtype=t.complex64, device=dev)
+    m.equalize_slots(w, y, llr, xhat=xh, **eq)
+    # the same slot equalized straight from the frequency domain
+    w2 = t.empty_like(csi)
+    m.zf(dev_c64(sl.csi.reshape(-1)), w2, B, U, G)
+    xh2 = t.zeros_like(xh)
+    m.equalize_slots(w2, dev_c64(sl.y.reshape(-1)), t.empty_like(llr), xhat=xh2, **eq)
+    t.cuda.synchronize()
+    x_chain = host_c64(xh).reshape(S, U, N)
+    x_direct = host_c64(xh2).reshape(S, U, N)
+    table = synth.qam_table(Q)
+    dec = np.abs(x_chain[..., None] - table).argmin(-1)
+    ref = np.abs(sl.x[..., None] - table).argmin(-1)
+    assert np.array_equal(dec, ref)
+    e_chain, e_direct = synth.evm_db(x_chain, sl.x), synth.evm_db(x_direct, sl.x)
+    assert e_chain < -25 and abs(e_chain - e_direct) < 0.5, (e_chain, e_direct)
