@@ -13,8 +13,8 @@
 // group are its (symbol, subcarrier) pairs, sg * S of them, since P_g serves
 // every symbol. One workgroup per (group, block of 64 antennas) stages that
 // block of P_g in LDS once ([U][64], at most 32 KB, coalesced loads); its four
-// waves take the group's 16-column tiles in turn, each computing 64 antennas x
-// 16 columns with A operands from LDS and B operands (user symbols) gathered
+// waves take the group's 16-column tiles in turn, each computing 32 antennas
+// (half the block; the two halves on alternate waves) x 16 columns with A operands from LDS and B operands (user symbols) gathered
 // from HBM. A complex product
 // takes real 16x16x4 steps with rows interleaved (antenna r re, r im) and K
 // interleaved (user u re, u im):
@@ -23,13 +23,16 @@
 // so D[2r][n] = Re out and D[2r+1][n] = Im out, and each lane ends with
 // whole complex outputs (rows 4*(l/16) .. +3 = two antennas) at column l%16.
 // FP32 accumulation, users in increasing order (two per step).
+#include <cstdlib>
+
 #include "eq_core.h"
 
 namespace mega {
 namespace detail {
 namespace {
 
-constexpr int kPrecAcc = 8;  // 8-antenna accumulator tiles per wave (64 antennas)
+// 8-antenna accumulator tiles per wave: NACC = 4 (32 antennas; the two
+// halves of a workgroup's 64 antennas on alternate waves) or 8 (64).
 
 // Workgroup = (group g, block of 64 antennas ab), ab fastest, in the logical
 // order of xcd_linear: consecutive workgroups share an XCD, so a group's user
@@ -38,10 +41,13 @@ constexpr int kPrecAcc = 8;  // 8-antenna accumulator tiles per wave (64 antenna
 // 16-column tiles w, w + 4, ... A tile's user symbols (at most UMAX users,
 // two per step) are all loaded before its products, so the HBM latency is
 // paid once per tile, not once per step.
-template <int UMAX>
+template <int UMAX, int NACC>
 __global__ __launch_bounds__(256) void precode_mfma(const PrecodeArgs a, uint32_t nab,
                                                    uint32_t ntile) {
   constexpr int NKS = UMAX / 2;
+  constexpr int WPT = 8 / NACC;  // waves sharing a tile (antenna halves)
+  const int wv = threadIdx.x >> 6;
+  const uint32_t bh = (uint32_t)(wv % WPT) * 8 * NACC;  // the wave's antenna offset
   __shared__ float2 sP[kMaxUsers * 64];
   const uint32_t wg = xcd_linear(blockIdx.x, gridDim.x);
   const uint32_t ab = wg % nab, g = wg / nab;
@@ -73,19 +79,19 @@ __global__ __launch_bounds__(256) void precode_mfma(const PrecodeArgs a, uint32_
   // -Im P (h = 0) or Im P (h = 1)
   const bool take_re = h == p;
   const float sgn = h ? 1.f : -1.f;
-  for (uint32_t t = threadIdx.x >> 6; t < ntile; t += 4) {
+  for (uint32_t t = wv / WPT; t < ntile; t += 4 / WPT) {
     float bv[NKS];
     load_x(t, bv);
-    v4f acc[kPrecAcc];
+    v4f acc[NACC];
 #pragma unroll
-    for (int m = 0; m < kPrecAcc; ++m) acc[m] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < NACC; ++m) acc[m] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
       if (2 * ks >= (int)U) break;  // uniform
       const uint32_t u = min(2 * ks + up, U - 1);  // u >= U: bv = 0 makes it inert
-      const float2* prow = sP + u * 64 + (i16 >> 1);
+      const float2* prow = sP + u * 64 + bh + (i16 >> 1);
 #pragma unroll
-      for (int m = 0; m < kPrecAcc; ++m) {
+      for (int m = 0; m < NACC; ++m) {
         const float2 pv = prow[8 * m];
         const float av = take_re ? pv.x : sgn * pv.y;
         acc[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[ks], acc[m], 0, 0, 0);
@@ -97,8 +103,8 @@ __global__ __launch_bounds__(256) void precode_mfma(const PrecodeArgs a, uint32_
       const uint64_t col = ((uint64_t)k + a.bs_start + a.row / 2) % a.row;
       float2* O = a.out + (uint64_t)s * a.out_sym + col;
 #pragma unroll
-      for (int m = 0; m < kPrecAcc; ++m) {
-        const uint32_t b = b0 + 8 * m + 2 * q;
+      for (int m = 0; m < NACC; ++m) {
+        const uint32_t b = b0 + bh + 8 * m + 2 * q;
         if (b < B) O[(uint64_t)b * a.row] = make_float2(acc[m][0], acc[m][1]);
         if (b + 1 < B) O[(uint64_t)(b + 1) * a.row] = make_float2(acc[m][2], acc[m][3]);
       }
@@ -122,12 +128,25 @@ hipError_t launch_precode(const PrecodeArgs& a, hipStream_t st, const char** why
     return hipErrorInvalidValue;
   }
   const dim3 grid((uint32_t)(G * nab)), block(256);
-  if (a.U <= 16)
-    hipLaunchKernelGGL(precode_mfma<16>, grid, block, 0, st, a, (uint32_t)nab, (uint32_t)ntile);
-  else if (a.U <= 32)
-    hipLaunchKernelGGL(precode_mfma<32>, grid, block, 0, st, a, (uint32_t)nab, (uint32_t)ntile);
-  else
-    hipLaunchKernelGGL(precode_mfma<64>, grid, block, 0, st, a, (uint32_t)nab, (uint32_t)ntile);
+  // 32 antennas per wave (fewer registers, one more wave per SIMD at 32
+  // users: 3% faster at 256x32); MEGA_PREC_ACC=8 selects 64
+  const char* e = std::getenv("MEGA_PREC_ACC");
+  const bool half = !(e && std::atoi(e) == 8);
+#define MEGA_PREC(UM)                                                                       \
+  if (half)                                                                                  \
+    hipLaunchKernelGGL((precode_mfma<UM, 4>), grid, block, 0, st, a, (uint32_t)nab,          \
+                       (uint32_t)ntile);                                                     \
+  else                                                                                       \
+    hipLaunchKernelGGL((precode_mfma<UM, 8>), grid, block, 0, st, a, (uint32_t)nab,          \
+                       (uint32_t)ntile);
+  if (a.U <= 16) {
+    MEGA_PREC(16)
+  } else if (a.U <= 32) {
+    MEGA_PREC(32)
+  } else {
+    MEGA_PREC(64)
+  }
+#undef MEGA_PREC
   return hipGetLastError();
 }
 
