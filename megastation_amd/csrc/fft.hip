@@ -18,7 +18,8 @@
 //
 // One workgroup of 256 threads per (antenna, symbol): samples are loaded
 // coalesced and scaled into LDS in bit-reversed order, ca/2 twiddles are
-// computed once into LDS (sincospif of exact arguments), and log2(ca)
+// computed once into LDS (sincospif of exact arguments for the first octant,
+// the rest by symmetry), and log2(ca)
 // radix-2 decimation-in-time stages run in place, four per barrier (radix
 // 2^4, sixteen elements per thread in registers); the store reads the
 // shifted bins straight from LDS. Sizes: powers of
@@ -108,11 +109,18 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     for (uint32_t n = threadIdx.x; n < N; n += kFftThreads)
       sx[pidx(__builtin_bitreverse32(n) >> (32 - L))] = load_iq(in, n, inv);
   }
-  const float dir = MODE == kIfft ? 2.0f : -2.0f;  // exp(+-2 pi i k / N)
-  for (uint32_t k = threadIdx.x; k < H; k += kFftThreads) {
+  // twiddles exp(-+2 pi i k / N) from the first octant: with c, s = cos,
+  // sin of 2 pi k / N (k <= N/8), W^k = (c, -+s), W^(N/4-k) = (s, -+c),
+  // W^(N/4+k) = (-s, -+c), W^(N/2-k) = (-c, -+s); coinciding indices fall
+  // to the same thread
+  const float sgn = MODE == kIfft ? 1.0f : -1.0f;
+  for (uint32_t k = threadIdx.x; k <= N / 8; k += kFftThreads) {
     float sn, cs;
-    sincospif(dir * (float)k / (float)N, &sn, &cs);
-    tw[k] = make_float2(cs, sn);
+    sincospif(2.0f * (float)k / (float)N, &sn, &cs);
+    tw[k] = make_float2(cs, sgn * sn);
+    tw[N / 4 - k] = make_float2(sn, sgn * cs);
+    tw[N / 4 + k] = make_float2(-sn, sgn * cs);
+    if (k > 0) tw[H - k] = make_float2(-cs, sgn * sn);
   }
   __syncthreads();
   // R stages per barrier (radix 2^R, R = 4, the last pass 1 to 3): a thread
