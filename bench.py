@@ -474,7 +474,7 @@ def main_learned(args):
     25 dB, 24-tap TDL (300 ns RMS) with 100 Hz Jakes Doppler, 0.5 ms slots.
     The network is trained first (200k group samples from two generator
     sequences, the library's kernels); a step is one slot of a held-out
-    sequence: learned ZF (Gram, network, 2 Newton-Schulz steps, judge,
+    sequence: learned ZF (Gram, network, 2 Newton-Schulz steps, residual check,
     plain-ZF fallback) + equalize/demod of 13 symbols. The plain FP32 form
     (ZF + equalize) on the same slots is timed beside it."""
     import torch

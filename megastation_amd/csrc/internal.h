@@ -55,7 +55,7 @@ hipError_t launch_detect_mixed(const EqualizeArgs& a, float2* w_out, void* work,
                                const char** why);
 
 // Learned form (learned.hip): Gram and plain inverse per group; network
-// features; warm start + Newton-Schulz + residual judge; final W with the
+// features; warm start + Newton-Schulz + residual check; final W with the
 // plain-sweep fallback. Matrices row-major [batch][U][U] complex.
 hipError_t launch_gram_inv(uint32_t U, uint32_t B, uint32_t batch, const float2* in, float2* G,
                            float2* X, int32_t* info, hipStream_t stream);

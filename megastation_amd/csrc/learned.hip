@@ -7,7 +7,7 @@
 //              (first-order perturbation of the inverse plus the network's
 //              correction C, Hermitian from its upper triangle);
 //   refine     n Newton-Schulz steps X <- X (2I - G_t X);
-//   judge      r = |X G_t - I|_F; groups with r above the bound (or not
+//   check      r = |X G_t - I|_F; groups with r above the bound (or not
 //              finite) are recomputed by the plain FP32 sweep (zf_core.h),
 //              the fallback;
 //   apply      W = X A on the matrix cores (zf_apply); X becomes X_{t-1}
@@ -141,7 +141,7 @@ __device__ __forceinline__ void cmm(const float2* A, const float2* B, Epi&& epi)
   }
 }
 
-// Warm start, Newton-Schulz refinement and the residual judge (users padded
+// Warm start, Newton-Schulz refinement and the residual check (users padded
 // to UP with zeros; the identity only on the first U diagonal entries).
 template <int UP>
 __global__ __launch_bounds__(256) void refine_kernel(
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void refine_kernel(
   }
 }
 
-// Final W = X A: groups flagged by the judge get X from the plain sweep of
+// Final W = X A: groups flagged by the residual check get X from the plain sweep of
 // their Gram matrix (the fallback), the others use the refined X. X (row-
 // major [U][U]) is updated in place for the next slot.
 template <int UT>

@@ -2,7 +2,7 @@
 predicts a correction to the first-order update of the previous slot's
 inverse, two Newton-Schulz steps refine it, and groups whose residual
 |W H - I|_F = |X G - I|_F exceeds the bound fall back to the plain FP32 ZF
-(which is also the judge of the error).
+(which also serves as the measure of the error).
 
 Network: features (2*U*(U+1): upper triangles of (G_t - G_{t-1})/s and
 X_{t-1}*s) -> 512 -> 512 -> U*(U+1) (upper triangle of the correction * s),
@@ -252,7 +252,7 @@ def features(Gm, Gp, Xp, U: int, stream=None, out=None):
 
 def refine(Gm, Gp, Xp, U: int, corr=None, scale=None, bound: float = 1e-3, iters: int = 2,
            stream=None, out=None):
-    """Warm start + Newton-Schulz + judge: (X, residual, flag)."""
+    """Warm start + Newton-Schulz + residual check: (X, residual, flag)."""
     import torch
     n = Gm.shape[0]
     if out is not None:

@@ -51,7 +51,7 @@ int orc_equalize_f32(uint64_t U, uint64_t B, uint64_t N, const float *csi,
                      uint64_t sg, int Q, uint64_t cb_size, uint64_t cb_stride,
                      uint64_t zeros, uint64_t valid_bits, float *xhat);
 
-/* Same in FP64 accumulation (the EVM judge); x_hat is rounded to float before
+/* Same in FP64 accumulation (the EVM reference); x_hat is rounded to float before
  * demod. xhat64 (optional): [U][N] complex128. */
 int orc_equalize_f64(uint64_t U, uint64_t B, uint64_t N, const float *csi,
                      const float *y, uint16_t *llr, uint64_t ue_stride,

@@ -111,7 +111,7 @@ def test_gram_inv_matches_fp64(U, B):
 
 
 @pytest.mark.parametrize("U,B", [(32, 128), (12, 48), (40, 160)])
-def test_refine_first_order_newton_schulz_and_judge(U, B):
+def test_refine_first_order_newton_schulz_and_residual_check(U, B):
     print(f"case: refine U={U} B={B}", flush=True)
     G = 12
     ch = synth.TdlJakes(B, U, G * 12, 12, seed=2)
@@ -182,7 +182,7 @@ def test_trained_network_reduces_fallbacks_on_held_out_sequence():
     sequence with and without the network: the network must cut both the
     median residual |X G - I|_F after two Newton-Schulz steps and the share
     of groups that fall back to the plain ZF; W of every group stays within
-    the judge's bound of the plain ZF's W."""
+    the residual check's bound of the plain ZF's W."""
     print("case: learned gain", flush=True)
     t = torch()
     U, B, N, sg = 32, 128, 3276, 12
@@ -203,7 +203,7 @@ def test_trained_network_reduces_fallbacks_on_held_out_sequence():
                 continue
             res.append(z.last["residual"].cpu().numpy())
             fb.append(z.last["flag"].cpu().numpy())
-            if ts % 10 == 0:   # the judge: plain ZF of the same slot
+            if ts % 10 == 0:   # the residual check: plain ZF of the same slot
                 Wz = t.empty_like(W)
                 m.zf(A, Wz, B, U, G)
                 a, b = host_c64(W).reshape(G, -1), host_c64(Wz).reshape(G, -1)
