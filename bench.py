@@ -499,6 +499,8 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(cfg, lay, csi, y, x, args.cpu_seconds)
     print(json.dumps(result), flush=True)
+    for p in list(pipes.values()) + list(chunk_pipes.values()):
+        p.close()
 
 
 def gen_y(A, cfg, seed):
