@@ -23,7 +23,7 @@
 // radix-2 decimation-in-time stages run in place, four per barrier (radix
 // 2^4, sixteen elements per thread in registers); the store reads the
 // shifted bins straight from LDS. Sizes: powers of
-// two from 16 to 4096 (50 KB of LDS at 4096).
+// two from 16 to 4096 (51 KB of LDS at 4096).
 #include "internal.h"
 
 namespace mega {
@@ -33,9 +33,10 @@ namespace {
 constexpr int kFftThreads = 256;
 constexpr uint32_t kFftMax = 4096;
 
-// LDS slot of element i: one pad element every 32, so the power-of-two
-// strides of the early passes (64 and 512 bytes across lanes) spread over
-// the banks.
+// LDS slot of element i of the samples and of the twiddle table: one pad
+// element every 32, so power-of-two strides across lanes (the samples' in the
+// early passes, the twiddles' in the later ones: up to 128 entries apart)
+// spread over the banks.
 __device__ __forceinline__ uint32_t pidx(uint32_t i) { return i + (i >> 5); }
 
 // u + w v and u - w v (complex)
@@ -65,7 +66,7 @@ __device__ __forceinline__ void fft_pass(float2* sx, const float2* tw, uint32_t 
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (m & sp) continue;
-        const float2 w = tw[(k + (uint32_t)(m & (sp - 1)) * half) << (L - 1 - st - q)];
+        const float2 w = tw[pidx((k + (uint32_t)(m & (sp - 1)) * half) << (L - 1 - st - q))];
         const float2 u = v[m], x = v[m + sp];
         v[m] = bfly_add(u, w, x);
         v[m + sp] = bfly_sub(u, w, x);
@@ -95,7 +96,7 @@ __device__ __forceinline__ Q quantize(float v) {
 template <typename Q, int MODE>
 __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   __shared__ float2 sx[kFftMax + kFftMax / 32];
-  __shared__ float2 tw[kFftMax / 2];
+  __shared__ float2 tw[kFftMax / 2 + kFftMax / 64];
   const uint32_t b = blockIdx.x, s = blockIdx.y;
   const uint32_t N = a.ca, L = a.log2ca, H = N / 2;
   const uint64_t row = (uint64_t)s * a.in_sym + (uint64_t)b * N;
@@ -117,10 +118,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   for (uint32_t k = threadIdx.x; k <= N / 8; k += kFftThreads) {
     float sn, cs;
     sincospif(2.0f * (float)k / (float)N, &sn, &cs);
-    tw[k] = make_float2(cs, sgn * sn);
-    tw[N / 4 - k] = make_float2(sn, sgn * cs);
-    tw[N / 4 + k] = make_float2(-sn, sgn * cs);
-    if (k > 0) tw[H - k] = make_float2(-cs, sgn * sn);
+    tw[pidx(k)] = make_float2(cs, sgn * sn);
+    tw[pidx(N / 4 - k)] = make_float2(sn, sgn * cs);
+    tw[pidx(N / 4 + k)] = make_float2(-sn, sgn * cs);
+    if (k > 0) tw[pidx(H - k)] = make_float2(-cs, sgn * sn);
   }
   __syncthreads();
   // R stages per barrier (radix 2^R, R = 4, the last pass 1 to 3): a thread
