@@ -49,6 +49,9 @@ def test_pilot_fft_reference_fixture(golden):
     (256, 4096, 3276, 410, 3, np.int16),  # 256-antenna cell, 4096-point
     (12, 512, 300, 100, 1, np.int8),      # char samples
     (5, 16, 16, 15, 2, np.int16),         # smallest size, every bin
+    (7, 64, 64, 3, 1, np.int16),          # last pass of two stages (6 = 4 + 2)
+    (4, 128, 100, 60, 2, np.int16),       # last pass of three stages (7 = 4 + 3)
+    (9, 1024, 600, 200, 2, np.int8),      # 10 = 4 + 4 + 2
 ])
 def test_uplink_fft_matches_oracle(B, ca, num, start, S, dtype):
     print(f"case: uplink FFT B={B} ca={ca} num={num} start={start} S={S} {dtype.__name__}",
