@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     fft_pass<1>(sx, tw, N, L, st);
     __syncthreads();
   }
-  const uint32_t shift = a.start + H;
+  const uint32_t shift = a.start + H;  // bins (i + shift) mod N, N a power of two
   if constexpr (MODE == kIfft) {
     Q* o = static_cast<Q*>(a.qout) + 2 * ((uint64_t)s * a.out_sym + (uint64_t)b * N);
     const float scale = a.qscale / (float)N;
@@ -154,12 +154,12 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     }
   } else if constexpr (MODE == kUplink) {
     float2* y = a.out + (uint64_t)s * a.out_sym + (uint64_t)b * a.num;
-    for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) y[i] = sx[pidx((i + shift) % N)];
+    for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) y[i] = sx[pidx((i + shift) & (N - 1))];
   } else {
     for (uint32_t i = threadIdx.x; i < a.num; i += kFftThreads) {
       const uint32_t u = a.ue_start + s * a.sg + i % a.sg;
       if (u >= a.U) continue;
-      const float2 e = sx[pidx((i + shift) % N)], p = a.pilot[i];
+      const float2 e = sx[pidx((i + shift) & (N - 1))], p = a.pilot[i];
       const uint64_t o = (uint64_t)(i / a.sg) * a.B * a.U + (uint64_t)b * a.U + u;
       a.out[(uint64_t)s * a.out_sym + o] =
           make_float2(e.x * p.x + e.y * p.y, e.x * p.y - e.y * p.x);
