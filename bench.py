@@ -2,26 +2,30 @@
 """Detected symbols/s of the uplink detection path (ZF + equalize + demod) on
 one MI355X.
 
-Workload (default): a 64-antenna x 16-user cell, 1200 data subcarriers in
-groups of 16 (75 groups), 13 data OFDM symbols per slot, 16-QAM at 20 dB SNR
-over i.i.d. CN(0,1) channels, batched over 200 slots. One step = ZF weights of
-all 200 x 75 groups + equalize/demod of all 200 x 13 symbols, inputs resident
-in HBM. Three forms are timed: the fused FP32 kernel (ZF and equalize in one
-launch, W kept on chip; up to 16 users), the drop-in plain FP32 form (ZF
-kernel writing W, then the equalize kernel) and the mixed form (split-BF16
-matrix-core ZF and equalize). Single-slot workloads (--workload c3_256x32)
-also time both forms slot-pipelined: ZF of step i+1 on a second stream, issued
-before the equalize of step i, so the two run together (W triple-buffered,
-event-ordered). The headline is
-the fastest form whose EVM on slot 0 is within 0.1 dB of FP64 zero-forcing's. Synthetic data generated on the
+Workload (default, c3_256x32): one 256-antenna x 32-user cell, 3276 data
+subcarriers in groups of 12 (273 groups), 13 data OFDM symbols per slot,
+64-QAM at 25 dB SNR over a Kronecker-correlated channel (rho_rx 0.5,
+rho_tx 0.3). One step = one slot: ZF weights of its 273 groups + equalize /
+demod of its 13 symbols, inputs resident in HBM (four rotating buffer sets so
+that the slot is not served from the Infinity Cache). Forms: the plain FP32
+drop-in (ZF kernel writing W, then the equalize kernel), the mixed form
+(split-BF16 matrix-core ZF and equalize), both slot-pipelined (ZF of step
+i+1 on a second stream beside the equalize of step i) and graph-captured
+with one or two slots per push, and the fused FP32 kernel where the shape
+allows. value = the fastest form whose arithmetic is FP32 throughout and
+whose EVM on slot 0 is within 0.1 dB of FP64 zero-forcing's (dtype fp32);
+the fastest form of all (the chosen form) and every form's ratio to the
+plain drop-in are reported beside it. A sub-record times the 64 x 16 cell
+batched over 200 slots (--workload c2_64x16_200slots runs it alone, with
+an EVM / BER sweep over SNR and QAM order). Synthetic data generated on the
 GPU with a seeded Philox generator.
 
 Prints one JSON line: value = detected symbols/s = U*N*S*slots / step time;
-roofline of the dominant kernel (equalize/demod, HBM-bound) from HIP events
-on the launch stream; the CPU baseline (the oracle's OpenMP restatement of
-the same operator math) timed on a bounded sample on the host cores; EVM of
-the GPU's equalized symbols next to FP64 zero-forcing's (numpy complex128)
-on a sample slot.
+roofline of the dominant kernel (the FP32 equalize/demod, HBM-bound) from HIP
+events on the launch stream; the CPU baseline (the oracle's OpenMP
+restatement of the same operator math) timed on a bounded sample on the host
+cores; EVM of the GPU's equalized symbols next to FP64 zero-forcing's (numpy
+complex128) on a sample slot.
 """
 import argparse
 import json
@@ -49,28 +53,15 @@ WORKLOADS = {
 }
 
 
-# Uneven batch: 40 cells, each drawing antenna x user, subcarriers, group
-# size and QAM order from these sets (seeded), 13 data symbols, i.i.d.
-# CN(0,1) channels at 20 dB SNR; one graph launch per step.
-BATCH_SHAPES = [(64, 8), (64, 16), (128, 16), (128, 32), (256, 64)]
-BATCH_N = [624, 1200, 3276]
-BATCH_SG = [12, 16]
-BATCH_Q = [4, 6, 8]
-
-
 def batch_cells(n_cells=40, seed=7):
-    rng = np.random.Generator(np.random.Philox(seed))
-    out = []
-    for _ in range(n_cells):
-        B, U = BATCH_SHAPES[rng.integers(len(BATCH_SHAPES))]
-        out.append(dict(U=U, B=B, N=int(BATCH_N[rng.integers(3)]),
-                        sg=int(BATCH_SG[rng.integers(2)]), S=13, slots=1,
-                        Q=int(BATCH_Q[rng.integers(3)]), snr=20.0, channel="iid"))
-    return out
+    """The uneven 40-cell batch (megastation_amd.synth.uneven_cells)."""
+    from megastation_amd import synth
+    return synth.uneven_cells(n_cells, seed)
 
 
 def gen_inputs(cfg, seed=1234):
-    """CSI [slots][G][B][U] (= H^H) and y [slots][S][B][N] on cuda:0."""
+    """CSI [slots][G][B][U] (= H^H), y [slots][S][B][N], the transmitted
+    symbols x [slots][S][U][N] and their indices (uint8) on cuda:0."""
     import torch
     from megastation_amd import synth
     U, B, N, sg, S, slots, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "slots", "Q"))
@@ -102,13 +93,17 @@ def gen_inputs(cfg, seed=1234):
          .reshape(slots, S, B, Np)[..., :N].contiguous())
     csi = torch.conj(H).resolve_conj().reshape(slots, G, B, U).contiguous()
     x = x[..., :N].contiguous()
-    return csi, y, x
+    return csi, y, x, idx[..., :N].to(torch.uint8)
 
 
 # Forms derived from a sequential one (same kernels, another schedule), and
 # the chunk counts of the chunked forms.
 DERIVED = ("_pipe", "_graph", "_graph2", "_chunk2", "_chunk4", "_chunk8")
 CHUNKS = (2, 4, 8)
+# Forms whose arithmetic is FP32 throughout (the reference's precision): the
+# headline value is the fastest of these; the mixed forms are the chosen
+# precision form, reported beside it with their ratio to the plain drop-in.
+FP32_FORMS = ("plain", "plain_pipe", "plain_graph", "plain_graph2", "fused")
 
 
 def main():
@@ -116,41 +111,69 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2_64x16_200slots",
+    ap.add_argument("--workload", default="c3_256x32",
                     choices=sorted(WORKLOADS) + ["c4_40cells", "c5_learned",
                                                  "d3_precode_256x32", "u3_fft_256x32"])
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="c3_256x32: skip the 64x16 x 200-slot sub-record")
+    ap.add_argument("--sweep-slots", type=int, default=20,
+                    help="c2_64x16_200slots: slots per SNR / QAM point of the EVM / BER sweep "
+                         "(0 = no sweep)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of a known size (1 GiB), whose "
                          "FETCH_SIZE / WRITE_SIZE counters scale the equalize traffic")
     args = ap.parse_args()
-    if args.workload == "c4_40cells" and args.gpus == 1:
-        return main_batch(args)
-    if args.workload == "c5_learned" and args.gpus == 1:
-        return main_learned(args)
-    if args.workload == "d3_precode_256x32" and args.gpus == 1:
-        return main_precode(args)
-    if args.workload == "u3_fft_256x32" and args.gpus == 1:
-        return main_fft(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the detection path shards by subcarrier group with no exchange; "
               "run one process per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4_40cells":
+        return main_batch(args)
+    if args.workload == "c5_learned":
+        return main_learned(args)
+    if args.workload == "d3_precode_256x32":
+        return main_precode(args)
+    if args.workload == "u3_fft_256x32":
+        return main_fft(args)
 
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    result = run_cell(args, args.workload, cpu=not args.no_cpu_baseline)
+    if args.workload == "c3_256x32" and not args.no_sub:
+        sub = run_cell(args, "c2_64x16_200slots", sel=("plain", "mixed"), cpu=False)
+        result["sub"] = {"c2_64x16_200slots": {
+            k: sub[k] for k in ("value", "ms_per_step", "dtype", "config", "roofline", "forms",
+                                "evm")}}
+    if args.workload == "c2_64x16_200slots" and args.sweep_slots > 0:
+        result["sweep"] = snr_sweep(args.sweep_slots)
+    print(json.dumps(result), flush=True)
+
+
+def run_cell(args, name, sel=None, cpu=True):
+    """Times the forms of one single-cell workload (all of them, or those in
+    sel) and returns the result record (value = fastest FP32 form)."""
     import torch
     import megastation_amd as m
     from megastation_amd import synth
 
-    cfg = WORKLOADS[args.workload]
+    cfg = WORKLOADS[name]
     U, B, N, sg, S, slots, Q = (cfg[k] for k in ("U", "B", "N", "sg", "S", "slots", "Q"))
     G = (N + sg - 1) // sg
     lay = synth.standalone_layout(N, Q)
     dev = torch.device("cuda:0")
-    print(f"[bench] workload {args.workload}: generating inputs", file=sys.stderr, flush=True)
-    csi0, y0, x = gen_inputs(cfg)
+    print(f"[bench] workload {name}: generating inputs", file=sys.stderr, flush=True)
+    csi0, y0, x, _ = gen_inputs(cfg)
     # Successive steps rotate through NR distinct sets of input and output
     # buffers, so that a one-slot working set (about 140 MB at 256x32, inside
     # the 256 MB Infinity Cache) is not served from that cache step after step:
@@ -161,7 +184,7 @@ def main():
     w_r = [torch.empty_like(csi0) for _ in range(NR)]
     llr_r = [torch.empty(slots * S * U * lay.ue_stride, dtype=torch.int16, device=dev)
              for _ in range(NR)]
-    csi, y, w, llr = csi0, y0, w_r[0], llr_r[0]
+    csi, y, w = csi0, y0, w_r[0]
     ring = [0]
 
     def nxt():
@@ -224,7 +247,7 @@ def main():
     # the ZF of step i+1 on its own stream before the equalize of step i, so
     # the two kernels run together (W triple-buffered, every dependency an
     # event, one C call per step).
-    from megastation_amd.pipeline import SlotPipeline
+    from megastation_amd.pipeline import PipelineGraph, SlotPipeline
     pipes = {}
 
     def get_pipe(mixed):
@@ -243,12 +266,10 @@ def main():
     # Graph forms (single-slot workloads): K consecutive slots (K = --steps,
     # buffer sets rotating as above) pipelined as SlotPipeline would and
     # captured into one graph (mega_pipeline_graph_*); one launch runs the K
-    # steps with no host work between them.
-    from megastation_amd.pipeline import PipelineGraph
-
-    # Two slots per push: pairs of consecutive slots in contiguous buffers
-    # (the four rotating slots as two pairs); the ZF of a pair is one launch
-    # over both slots' groups, its equalize one launch.
+    # steps with no host work between them. Two slots per push: pairs of
+    # consecutive slots in contiguous buffers (the four rotating slots as two
+    # pairs); the ZF of a pair is one launch over both slots' groups, its
+    # equalize one launch.
     pair_r = []
 
     def pairs():
@@ -283,14 +304,6 @@ def main():
         t1.record(stream)
         torch.cuda.synchronize()
         return t0.elapsed_time(t1) / steps, None, None
-
-    if args.copy_calibration:
-        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     host_ms = {}  # host time to issue one pipelined step (launches, waits, records)
 
@@ -328,23 +341,28 @@ def main():
         k2 = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
         return ms, k1, k2
 
+    def want(k):
+        return sel is None or k in sel
+
     forms = {}
-    forms["fused"] = timed(step_fused) if m.detect_fused_shape(U, B) else None
-    print(f"[bench] fused form timed ({args.steps} steps)", file=sys.stderr, flush=True)
-    forms["plain"] = timed(step_plain)
-    print(f"[bench] plain form timed ({args.steps} steps)", file=sys.stderr, flush=True)
-    forms["mixed"] = timed(step_mixed)
-    print(f"[bench] mixed form timed ({args.steps} steps)", file=sys.stderr, flush=True)
+    if want("fused") and m.detect_fused_shape(U, B):
+        forms["fused"] = timed(step_fused)
+    for k, fn in (("plain", step_plain), ("mixed", step_mixed)):
+        if want(k):
+            forms[k] = timed(fn)
+    print(f"[bench] {name}: sequential forms timed ({args.steps} steps)", file=sys.stderr,
+          flush=True)
     if slots == 1:
-        for name, mixed in (("plain_pipe", False), ("mixed_pipe", True)):
-            forms[name] = timed(lambda evs=None, mx=mixed: step_pipe(mx), pipe=get_pipe(mixed))
-            print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
-        for name, mixed, per in (("plain_graph", False, 1), ("mixed_graph", True, 1),
-                                 ("plain_graph2", False, 2), ("mixed_graph2", True, 2)):
-            gr, steps = make_graph(mixed, per)
-            forms[name] = timed_graph(gr, steps)
-            gr.close()
-            print(f"[bench] {name} form timed ({args.steps} steps)", file=sys.stderr, flush=True)
+        for k, mixed in (("plain_pipe", False), ("mixed_pipe", True)):
+            if want(k):
+                forms[k] = timed(lambda evs=None, mx=mixed: step_pipe(mx), pipe=get_pipe(mixed))
+        for k, mixed, per in (("plain_graph", False, 1), ("mixed_graph", True, 1),
+                              ("plain_graph2", False, 2), ("mixed_graph2", True, 2)):
+            if want(k):
+                gr, steps = make_graph(mixed, per)
+                forms[k] = timed_graph(gr, steps)
+                gr.close()
+        print(f"[bench] {name}: pipelined and graph forms timed", file=sys.stderr, flush=True)
 
     # Chunked forms (multi-slot workloads): the job's slots go through the
     # slot pipeline in C pushes of slots / C, so the ZF of chunk j+1 (bound by
@@ -371,7 +389,7 @@ def main():
 
     if slots > 1:
         for C in CHUNKS:
-            if slots % C:
+            if slots % C or not want(f"mixed_chunk{C}"):
                 continue
             forms[f"mixed_chunk{C}"] = timed(lambda evs=None, C=C: step_chunk(C),
                                              pipe=get_chunk_pipe(C))
@@ -402,13 +420,13 @@ def main():
     llr1 = torch.empty(S * U * lay.ue_stride, dtype=torch.int16, device=dev)
     x0 = x[0].cpu().numpy()
     evm = {}
-    for name in forms:
-        if forms[name] is None or name.endswith(DERIVED):
+    for k in forms:
+        if k.endswith(DERIVED):
             continue
         xh.zero_()
-        if name == "fused":
+        if k == "fused":
             m.detect_slots(csi[0], y[0], llr1, xhat=xh, stream=stream, **kw1)
-        elif name == "plain":
+        elif k == "plain":
             m.zf(csi[0], w[0], B, U, G, stream=stream)
             m.equalize_slots(w[0], y[0], llr1, xhat=xh, stream=stream, **kw1)
         else:
@@ -416,37 +434,38 @@ def main():
             m.equalize_slots(split, y[0], llr1, xhat=xh, form=m.FORM_MIXED,
                              stream=stream, **dict(kw1, csi_slot_stride=G * U * Bp))
         torch.cuda.synchronize()
-        evm[name] = synth.evm_db(xh.cpu().numpy().reshape(S, U, N), x0)
-    for name in forms:
-        if name.endswith(DERIVED):  # same kernels as the sequential form
-            evm[name] = evm[name.rsplit("_", 1)[0]]
-    evm64 = fp64_evm(cfg, csi, y, x0)
+        evm[k] = synth.evm_db(xh.cpu().numpy().reshape(S, U, N), x0)
+    for k in forms:
+        if k.endswith(DERIVED):  # same kernels as the sequential form
+            evm[k] = evm[k.rsplit("_", 1)[0]]
+    evm64 = synth.evm_db(fp64_xhat(cfg, csi[:1], y[:1])[0], x0)
     ok = {k: abs(v - evm64) < 0.1 and chunk_same.get(k, True) for k, v in evm.items()}
 
-    # Headline = the fastest form whose EVM is within 0.1 dB of FP64's.
-    cands = [k for k in forms if forms[k] is not None and ok.get(k)]
-    head_form = min(cands, key=lambda k: forms[k][0]) if cands else "plain"
+    # value = the fastest FP32 form within 0.1 dB of FP64's EVM; the chosen
+    # (fastest of all) form and every form's ratio to the plain drop-in beside it
+    fp32 = [k for k in forms if k in FP32_FORMS and ok.get(k)]
+    head_form = min(fp32, key=lambda k: forms[k][0]) if fp32 else "plain"
+    anyf = [k for k in forms if ok.get(k)]
+    chosen = min(anyf, key=lambda k: forms[k][0]) if anyf else head_form
     ms_step = forms[head_form][0]
+    ms_plain = forms["plain"][0] if "plain" in forms else None
     if head_form == "fused":
         dom_kernel, dom_ms, dom_prefix = ("detect_fused (ZF + equalize + demod)",
                                           forms["fused"][1], "detect_fused")
-    elif head_form.startswith("plain"):
-        dom_kernel, dom_ms, dom_prefix = "equalize_demod", forms["plain"][2], "equalize_"
     else:
-        dom_kernel, dom_ms, dom_prefix = ("equalize_demod (split BF16)", forms["mixed"][2],
-                                          "equalize_mixed")
+        dom_kernel, dom_ms, dom_prefix = "equalize_demod (FP32)", forms["plain"][2], "equalize_"
     value = symbols / (ms_step * 1e-3)
     achieved = io_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = measured_traffic(args.workload, dom_prefix)
-    print(f"[bench] headline {head_form}: {ms_step:.3f} ms/step; " +
-          ", ".join(f"{k} {v[0]:.3f}" for k, v in forms.items() if v),
-          file=sys.stderr, flush=True)
+    traffic = measured_traffic(name, dom_prefix)
+    print(f"[bench] {name}: value {head_form} {ms_step:.4f} ms/step, chosen {chosen} "
+          f"{forms[chosen][0]:.4f}; " +
+          ", ".join(f"{k} {v[0]:.4f}" for k, v in forms.items()), file=sys.stderr, flush=True)
 
     def form_entry(k):
-        if forms[k] is None:
-            return None
         ms, t1, t2 = forms[k]
         e = {"symbols_per_s": symbols / (ms * 1e-3), "ms_per_step": ms,
+             "arith": "fp32" if k in FP32_FORMS else "bf16x3 split, fp32 accumulate",
+             "ratio_to_plain": ms_plain / ms if ms_plain else None,
              "evm_db": evm[k], "evm_delta_db": evm[k] - evm64, "evm_ok": ok[k]}
         if k.endswith("_pipe"):
             e["schedule"] = ("ZF of step i+1 on a second stream, issued before the "
@@ -470,6 +489,7 @@ def main():
             e.update({"zf_ms": t1, "equalize_ms": t2,
                       "zf_tflops": zf_flops / (t1 * 1e-3) / 1e12,
                       "equalize_gbs": io_bytes / (t2 * 1e-3) / 1e9,
+                      "equalize_hbm_frac": io_bytes / (t2 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                       "equalize_tflops": eq_flops / (t2 * 1e-3) / 1e12})
         return e
 
@@ -477,9 +497,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "symbols/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16x3 split, fp32 accumulate" if head_form.startswith("mixed") else "fp32",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "antennas": B, "users": U,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": name, "antennas": B, "users": U,
                    "subcarriers": N, "sc_group": sg, "groups": G, "data_symbols": S,
                    "slots": slots, "qam": 1 << Q, "snr_db": cfg["snr"],
                    "channel": cfg["channel"], "form": head_form,
@@ -490,17 +509,80 @@ def main():
                      "traffic": traffic["traffic_bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
                      "algorithmic_bytes": io_bytes, "avg_ms": dom_ms},
+        "chosen_form": {"form": chosen, "ms_per_step": forms[chosen][0],
+                        "symbols_per_s": symbols / (forms[chosen][0] * 1e-3),
+                        "ratio_to_plain": ms_plain / forms[chosen][0] if ms_plain else None,
+                        "ratio_to_value": ms_step / forms[chosen][0]},
         "forms": {k: form_entry(k) for k in forms},
         "evm": {"form": head_form, "gpu_db": evm[head_form], "fp64_db": evm64,
                 "delta_db": evm[head_form] - evm64,
                 "within_0p1_db": ok[head_form]},
     }
-
-    if not args.no_cpu_baseline:
+    if cpu:
         result["cpu_baseline"] = cpu_baseline(cfg, lay, csi, y, x, args.cpu_seconds)
-    print(json.dumps(result), flush=True)
     for p in list(pipes.values()) + list(chunk_pipes.values()):
         p.close()
+    return result
+
+
+def snr_sweep(n_slots):
+    """EVM and BER of the plain and mixed forms on the 64 x 16 cell (1200
+    subcarriers, groups of 16, 13 symbols, i.i.d. channels) at SNR 10-30 dB
+    in 5 dB steps, 16- and 64-QAM, n_slots slots per point, next to FP64
+    zero-forcing (numpy complex128) on the same inputs. BER counts hard
+    decisions (LLR sign, LLR > 0 = bit 0) against the transmitted bits;
+    'vs_fp64' counts them against the hard decisions of the FP64 x_hat."""
+    import torch
+    import megastation_amd as m
+    from megastation_amd import synth
+    out = []
+    dev = torch.device("cuda:0")
+    for Q in (4, 6):
+        for snr in (10.0, 15.0, 20.0, 25.0, 30.0):
+            cfg = dict(WORKLOADS["c2_64x16_200slots"], slots=n_slots, Q=Q, snr=snr)
+            U, B, N, sg, S, slots = (cfg[k] for k in ("U", "B", "N", "sg", "S", "slots"))
+            G = (N + sg - 1) // sg
+            Bp = (B + 15) // 16 * 16
+            lay = synth.standalone_layout(N, Q)
+            csi, y, x, idx = gen_inputs(cfg, seed=int(100 * snr) + Q)
+            kw = dict(num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, num_slots=slots,
+                      mod_order=Q, ue_stride=lay.ue_stride, cb_size=lay.cb_size,
+                      cb_stride=lay.cb_stride, zeros=lay.zeros, valid_bits=lay.valid_bits,
+                      csi_slot_stride=G * B * U, y_sym_stride=B * N, y_slot_stride=S * B * N,
+                      llr_sym_stride=U * lay.ue_stride, llr_slot_stride=S * U * lay.ue_stride)
+            llr = torch.empty(slots * S * U * lay.ue_stride, dtype=torch.int16, device=dev)
+            xh = torch.empty(slots * S * U * N, dtype=torch.complex64, device=dev)
+            bits = synth.index_bits(idx.cpu().numpy(), Q)              # [slots][S][U][N][Q]
+            x64 = fp64_xhat(cfg, csi, y)
+            h64 = synth.hard_bits_xhat(x64, Q)
+            xn = x.cpu().numpy()
+            rec = {"qam": 1 << Q, "snr_db": snr, "slots": slots, "bits": int(bits.size),
+                   "fp64": {"evm_db": synth.evm_db(x64, xn),
+                            "ber": float(np.mean(h64 != bits))}}
+            for form in ("plain", "mixed"):
+                if form == "plain":
+                    w = torch.empty_like(csi)
+                    m.zf(csi, w, B, U, slots * G)
+                    m.equalize_slots(w, y, llr, xhat=xh, **kw)
+                else:
+                    sp = torch.empty(slots * G * U * Bp, dtype=torch.int64, device=dev)
+                    m.zf_split(csi, sp, B, U, slots * G)
+                    m.equalize_slots(sp, y, llr, xhat=xh, form=m.FORM_MIXED,
+                                     **dict(kw, csi_slot_stride=G * U * Bp))
+                torch.cuda.synchronize()
+                l16 = llr.cpu().numpy().view(np.uint16).reshape(slots, S, U, lay.ue_stride)
+                hb = synth.hard_bits_llr(l16[..., :N * Q].reshape(slots, S, U, N, Q))
+                xg = xh.cpu().numpy().reshape(slots, S, U, N)
+                rec[form] = {"evm_db": synth.evm_db(xg, xn),
+                             "ber": float(np.mean(hb != bits)),
+                             "vs_fp64": float(np.mean(hb != h64))}
+            rec["evm_delta_db"] = {f: rec[f]["evm_db"] - rec["fp64"]["evm_db"]
+                                   for f in ("plain", "mixed")}
+            out.append(rec)
+            print(f"[bench] sweep {1 << Q}-QAM {snr:.0f} dB: BER fp64 {rec['fp64']['ber']:.3g} "
+                  f"plain {rec['plain']['ber']:.3g} mixed {rec['mixed']['ber']:.3g}",
+                  file=sys.stderr, flush=True)
+    return out
 
 
 def gen_y(A, cfg, seed):
@@ -531,8 +613,9 @@ def main_learned(args):
     """The learned form over a slot sequence of the time-varying channel:
     128 x 32 antennas x users, 3276 subcarriers in groups of 12, 64-QAM at
     25 dB, 24-tap TDL (300 ns RMS) with 100 Hz Jakes Doppler, 0.5 ms slots.
-    The network is trained first (200k group samples from two generator
-    sequences, the library's kernels); a step is one slot of a held-out
+    The network is the committed checkpoint (200k slot pairs of two generator
+    sequences, trained by the library's kernels, megastation_amd.train_learned;
+    trained here on 200k group samples if absent); a step is one slot of a held-out
     sequence: learned ZF (Gram, network, 2 Newton-Schulz steps, residual check,
     plain-ZF fallback) + equalize/demod of 13 symbols. The plain FP32 form
     (ZF + equalize) on the same slots is timed beside it."""
@@ -544,14 +627,21 @@ def main_learned(args):
     G = (N + sg - 1) // sg
     lay = synth.standalone_layout(N, Q)
     dev = torch.device("cuda:0")
+    from megastation_amd import train_learned
     t0 = time.perf_counter()
-    chans = [learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=s)) for s in (11, 12)]
-    net = learned.WarmStartNet(U, seed=0)
-    hist = learned.train(net, chans, samples=200_000)
+    if os.path.exists(train_learned.DEFAULT_OUT):
+        # the committed checkpoint (200k generator slot pairs), SHA-256 checked
+        net, tinfo = learned.load(train_learned.DEFAULT_OUT)
+        training = dict(tinfo, source=os.path.relpath(train_learned.DEFAULT_OUT, ROOT))
+    else:
+        chans = [learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=s)) for s in (11, 12)]
+        net = learned.WarmStartNet(U, seed=0)
+        hist = learned.train(net, chans, samples=200_000)
+        training = {"source": "trained in this run", "train_samples": hist[-1][0],
+                    "loss_first": hist[0][1], "loss_last": hist[-1][1]}
     torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    print(f"[bench] learned: trained in {train_s:.1f} s, loss {hist[0][1]:.4g} -> {hist[-1][1]:.4g}",
-          file=sys.stderr, flush=True)
+    training["load_or_train_seconds"] = time.perf_counter() - t0
+    print(f"[bench] learned: {training}", file=sys.stderr, flush=True)
     test = learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=99))
     n_slots = args.warmup + args.steps
     slots = []
@@ -624,8 +714,7 @@ def main_learned(args):
         "forms": {"learned": {"ms_per_step": ms_learned, "fallback_rate": fb_rate},
                   "plain": {"ms_per_step": ms_plain,
                             "symbols_per_s": symbols / (ms_plain * 1e-3)}},
-        "training": {"samples": hist[-1][0], "seconds": train_s, "loss_first": hist[0][1],
-                     "loss_last": hist[-1][1]},
+        "training": training,
         "evm": {"gpu_db": evm_gpu, "fp64_db": evm64, "delta_db": evm_gpu - evm64,
                 "within_0p1_db": abs(evm_gpu - evm64) < 0.1},
     }
@@ -830,7 +919,7 @@ def main_batch(args):
     for k, cfg in enumerate(cfgs):
         U, B, N, sg, S, Q = (cfg[x] for x in ("U", "B", "N", "sg", "S", "Q"))
         G = (N + sg - 1) // sg
-        csi, y, x = gen_inputs(cfg, seed=1000 + k)
+        csi, y, x, _ = gen_inputs(cfg, seed=1000 + k)
         lay = synth.standalone_layout(N, Q)
         cells.append(dict(U=U, B=B, N=N, sg=sg, S=S, Q=Q, layout=lay, csi=csi[0], y=y[0], x=x[0],
                           w=torch.empty(G * B * U, dtype=torch.complex64, device=dev),
@@ -850,6 +939,32 @@ def main_batch(args):
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
+    # The same 40 cells through the single-cell drop-in entry points, one
+    # cell after another (mega_zf, then mega_equalize_slots for its 13
+    # symbols): what a caller without the batching form would launch.
+
+    def per_cell():
+        for c in cells:
+            U, B, N, sg, S, Q, lay = (c[x] for x in ("U", "B", "N", "sg", "S", "Q", "layout"))
+            G = (N + sg - 1) // sg
+            m.zf(c["csi"], c["w"], B, U, G, stream=stream)
+            m.equalize_slots(c["w"], c["y"], c["llr"], num_ue=U, num_bs=B, num_sc=N,
+                             sc_group=sg, num_syms=S, num_slots=1, mod_order=Q,
+                             ue_stride=lay.ue_stride, cb_size=lay.cb_size,
+                             cb_stride=lay.cb_stride, zeros=lay.zeros,
+                             valid_bits=lay.valid_bits, csi_slot_stride=G * B * U,
+                             y_sym_stride=B * N, y_slot_stride=S * B * N,
+                             llr_sym_stride=U * lay.ue_stride,
+                             llr_slot_stride=S * U * lay.ue_stride, stream=stream)
+    for _ in range(args.warmup):
+        per_cell()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        per_cell()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_cell = e0.elapsed_time(e1) / args.steps
     symbols = sum(c["U"] * c["N"] * c["S"] for c in cells)
     io = sum(8 * c["B"] * c["N"] * c["S"] + 3 * 8 * c["U"] * c["B"] * ((c["N"] + c["sg"] - 1) // c["sg"])
              + 2 * c["U"] * c["Q"] * c["N"] * c["S"] for c in cells)
@@ -885,6 +1000,11 @@ def main_batch(args):
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": io, "avg_ms": ms},
+        "forms": {"batch_graph": {"ms_per_step": ms, "symbols_per_s": symbols / (ms * 1e-3)},
+                  "per_cell": {"ms_per_step": ms_cell, "symbols_per_s": symbols / (ms_cell * 1e-3),
+                               "schedule": "mega_zf + mega_equalize_slots per cell, cells in "
+                                           "sequence on one stream"},
+                  "batch_gain": ms_cell / ms},
         "plan": batch.plan(), "plan_seconds": plan_s,
         "evm": {"cell": 0, "gpu_db": evm_gpu, "fp64_db": evm64, "delta_db": evm_gpu - evm64,
                 "within_0p1_db": abs(evm_gpu - evm64) < 0.1},
@@ -949,22 +1069,29 @@ def cpu_baseline(cfg, lay, csi, y, x, budget_s):
     return base
 
 
-def fp64_evm(cfg, csi, y, x0):
-    """EVM (dB) of FP64 zero-forcing of slot 0 against the transmitted
-    symbols: the reference every form's EVM is held to. Textbook ZF in numpy
-    complex128 (W_g = (A_g A_g^H)^-1 A_g by LAPACK solve, then x = W_g y per
-    subcarrier), independent of the library."""
-    from megastation_amd import synth
+def fp64_xhat(cfg, csi, y):
+    """x_hat of FP64 zero-forcing, complex128 [slots][S][U][N], for csi
+    [slots][G][B][U] and y [slots][S][B][N] (torch, any device): textbook ZF
+    in numpy (W_g = (A_g A_g^H)^-1 A_g by LAPACK solve, then x = W_g y per
+    subcarrier), independent of the library. The reference every form's EVM
+    and hard decisions are held to."""
     U, B, N, sg, S = (cfg[k] for k in ("U", "B", "N", "sg", "S"))
     G = (N + sg - 1) // sg
-    A = csi[0].cpu().numpy().astype(np.complex128).reshape(G, B, U).transpose(0, 2, 1)
-    W = np.linalg.solve(A @ A.conj().transpose(0, 2, 1), A)  # [G][U][B]
-    y0 = y[0].cpu().numpy().astype(np.complex128).reshape(S, B, N)
-    yp = np.zeros((S, B, G * sg), np.complex128)
-    yp[:, :, :N] = y0
-    Y = yp.reshape(S, B, G, sg).transpose(0, 2, 1, 3)  # [S][G][B][sg]
-    x64 = (W[None] @ Y).transpose(0, 2, 1, 3).reshape(S, U, G * sg)[:, :, :N]
-    return synth.evm_db(x64, x0)
+    n = csi.shape[0]
+    A = csi.cpu().numpy().astype(np.complex128).reshape(n, G, B, U).transpose(0, 1, 3, 2)
+    W = np.linalg.solve(A @ A.conj().transpose(0, 1, 3, 2), A)   # [n][G][U][B]
+    yp = np.zeros((n, S, B, G * sg), np.complex128)
+    yp[..., :N] = y.cpu().numpy().reshape(n, S, B, N)
+    Y = yp.reshape(n, S, B, G, sg).transpose(0, 3, 2, 1, 4).reshape(n, G, B, S * sg)
+    X = W @ Y                                                    # [n][G][U][S*sg]
+    return X.reshape(n, G, U, S, sg).transpose(0, 3, 2, 1, 4).reshape(n, S, U, G * sg)[..., :N]
+
+
+def fp64_evm(cfg, csi, y, x0):
+    """EVM (dB) of FP64 zero-forcing of slot 0 against the transmitted
+    symbols x0 [S][U][N]."""
+    from megastation_amd import synth
+    return synth.evm_db(fp64_xhat(cfg, csi[:1], y[:1])[0], x0)
 
 
 if __name__ == "__main__":

@@ -53,6 +53,13 @@ typedef struct mega_beamform *mega_beamform_t;
 const char *mega_version(void);
 const char *mega_last_error(void);
 
+/* Re-reads the MEGA_* tuning overrides from the environment. The launchers
+ * parse them once, at their first use, and keep them; this makes a later
+ * change of the environment take effect (tuning sweeps, tests). Values
+ * outside a variable's accepted range mean the launcher's own choice. No
+ * reference counterpart. Always MEGA_OK. */
+int mega_tuning_reload(void);
+
 /* ---- Beamform ------------------------------------------------------------ */
 
 /* Replaces Beamform::Beamform(num_bs, num_ue, batch_count)
@@ -419,10 +426,9 @@ int mega_learned_finish(uint32_t num_bs, uint32_t num_ue, uint32_t batch, const 
  * B(k,j) = B[k*sbk + j*sbj]; C row-major (ldc) in FP32 (Cf, pre-activation,
  * Cf = z + beta*Cf) and/or BF16 (Cb, after the activation). Small M x N
  * grids split K: the partial sums go through a library-owned workspace per
- * device (allocated on first need, grown with hipMalloc), so GEMMs that
- * overlap on one device (different streams) must be serialized by the
- * caller, and the call must not be captured into a graph when it has to
- * grow the workspace. */
+ * (device, stream), allocated on first need and grown with hipMalloc, so
+ * GEMMs on different streams may overlap; such a split GEMM is refused
+ * (MEGA_ERR_HIP) while its stream is being captured into a graph. */
 typedef struct {
   const void *A;
   int64_t sai, sak;

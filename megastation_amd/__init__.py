@@ -122,6 +122,7 @@ def lib() -> ctypes.CDLL:
     sig = {
         "mega_version": ([], ctypes.c_char_p),
         "mega_last_error": ([], ctypes.c_char_p),
+        "mega_tuning_reload": ([], i32),
         "mega_beamform_create": ([u32, u32, u32, ctypes.POINTER(P)], i32),
         "mega_beamform_destroy": ([P], i32),
         "mega_beamform_uplink": ([P, P, u32, P], i32),
@@ -174,6 +175,12 @@ def _check(rc: int) -> int:
 
 def last_error() -> str:
     return lib().mega_last_error().decode()
+
+
+def tuning_reload() -> None:
+    """Make changed MEGA_* tuning variables take effect (the library parses
+    them once and keeps them)."""
+    lib().mega_tuning_reload()
 
 
 def _stream(stream):

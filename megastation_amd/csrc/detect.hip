@@ -79,10 +79,7 @@ hipError_t launch_one(const EqParams& p, dim3 grid, size_t lds, hipStream_t st) 
 }
 
 // MEGA_DET_PF=0 turns the next-tile y prefetch off (tuning sweeps).
-bool detect_prefetch() {
-  const char* e = std::getenv("MEGA_DET_PF");
-  return !(e && e[0] == '0');
-}
+bool detect_prefetch() { return tune(kDetPf) != 0; }
 
 template <int MT, int NW>
 hipError_t launch_q(const EqParams& p, dim3 grid, size_t lds, uint32_t Q,

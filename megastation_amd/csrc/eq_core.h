@@ -415,6 +415,13 @@ inline bool eq_stream_ok(uint32_t B, int KC, int R) { return B % (4 * KC * R) ==
 // (equalize.hip).
 hipError_t fill_eq_params(const EqualizeArgs& a, EqParams& p, const char** why);
 
+// Wide-cell FP32 form (equalize_wide.hip): antennas split over the four
+// waves of a workgroup, W slices in registers, three-product complex MFMAs.
+// Shapes it takes: up to 32 users, 33 to 256 antennas.
+bool eq_wide_ok(uint32_t U, uint32_t B);
+hipError_t launch_equalize_wide(const EqParams& p, uint32_t tc_override, hipStream_t st,
+                                const char** why);
+
 // Host-side tile plan shared by the equalize and detect launchers.
 struct EqPlan {
   uint32_t MT, KS, nmg, nwk;

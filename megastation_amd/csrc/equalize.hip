@@ -121,11 +121,6 @@ hipError_t launch_q(const EqParams& p, dim3 grid, dim3 block, size_t lds,
   }
 }
 
-int env_int(const char* name) {
-  const char* e = std::getenv(name);
-  return e ? std::atoi(e) : 0;
-}
-
 }  // namespace
 
 hipError_t fill_eq_params(const EqualizeArgs& a, EqParams& p, const char** why) {
@@ -173,15 +168,21 @@ hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t st,
   EqParams p;
   hipError_t e = fill_eq_params(a, p, why);
   if (e != hipSuccess) return e;
+  // Wide cells (17 to 32 users, 128 to 256 antennas, e.g. 256 x 32): the
+  // antennas split over a workgroup's waves with W in registers
+  // (equalize_wide.hip); MEGA_EQ_MODE=3 forces that form wherever it applies.
+  const int mode = tune(kEqMode);
+  if (eq_wide_ok(a.U, a.B) &&
+      (mode == 3 || (mode == 0 && a.U > 16 && a.B >= 128)))
+    return launch_equalize_wide(p, (uint32_t)tune(kEqWtc), st, why);
   // Tile plan; MEGA_EQ_KS=8|16 overrides the antenna step count for sweeps.
-  const EqPlan pl = plan_equalize(a.U, a.B, (uint32_t)env_int("MEGA_EQ_KS"));
+  const EqPlan pl = plan_equalize(a.U, a.B, (uint32_t)tune(kEqKs));
   // W held in registers while one wave covers all antennas; otherwise W is
   // streamed (MEGA_EQ_MODE=1 forces streaming, 2 forbids it).
-  const int mode = env_int("MEGA_EQ_MODE");
-  const int sh = env_int("MEGA_EQ_SCH") % 3;
+  const int sh = tune(kEqSch);  // 0..2
   if (mode != 2 && (mode == 1 || pl.nwk > 1) && eq_stream_ok(a.B, kStreamKC[sh], kStreamR[sh])) {
     uint32_t MT = a.U <= 8 ? 1 : a.U <= 16 ? 2 : 4;
-    const int smt = env_int("MEGA_EQ_SMT");  // users per wave / 8 (sweeps)
+    const int smt = tune(kEqSmt);  // users per wave / 8 (sweeps)
     if (smt == 1 || smt == 2 || smt == 4) MT = (uint32_t)smt;
     p.nmg = (a.U + 8 * MT - 1) / (8 * MT);
     p.nwk = 1;
@@ -210,7 +211,7 @@ hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t st,
   uint32_t chunks = 1;
   while (units * chunks < 2048 && chunks < p.col_tiles) ++chunks;
   p.tiles_per_wg = (p.col_tiles + chunks - 1) / chunks;
-  const int tpw = env_int("MEGA_EQ_TPW");
+  const int tpw = tune(kEqTpw);
   if (tpw >= 1) p.tiles_per_wg = (uint32_t)tpw < p.col_tiles ? (uint32_t)tpw : p.col_tiles;
   p.chunks = (p.col_tiles + p.tiles_per_wg - 1) / p.tiles_per_wg;
   const uint64_t nblocks = units * p.chunks * pl.nmg;

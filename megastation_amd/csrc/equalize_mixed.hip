@@ -380,11 +380,6 @@ hipError_t launch_mt(const EqParams& p, const uint32_t* wp, uint32_t units, uint
   }
 }
 
-int env_int(const char* name) {
-  const char* e = std::getenv(name);
-  return e ? std::atoi(e) : 0;
-}
-
 }  // namespace
 
 // a.csi is ignored: the weights come from wp (split planes of mega_zf_split;
@@ -399,14 +394,14 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStrea
   // 16-user tiles per wave: 2 when there are more than 16 users (one y read
   // for 32 users), else 1; MEGA_EQ_XMT=1|2 overrides (tuning sweeps).
   uint32_t MT = a.U <= 16 ? 1 : 2;
-  const int smt = env_int("MEGA_EQ_XMT");
+  const int smt = tune(kEqXmt);
   if (smt == 1 || smt == 2) MT = (uint32_t)smt;
   // W in registers when the wave's W fits in 64 registers (split rows of at
   // most 128 antennas; MEGA_EQ_XREG=1 forces streaming): W is read once
   // per wave for a chunk of the group's tiles, chunks sized so the grid has
   // at least 4096 waves.
   const uint32_t nsw = split_row(a.B) / 16;
-  const int xreg = env_int("MEGA_EQ_XREG");
+  const int xreg = tune(kEqXreg);
   if (xreg != 1 && (nsw == 4 || nsw == 8) && MT * nsw <= 8) {
     p.nmg = (a.U + 16 * MT - 1) / (16 * MT);
     p.nwk = 1;
@@ -429,7 +424,7 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStrea
   // Column tiles per wave: 2 (one W stream for 32 columns) when a group has
   // more than one tile, else 1; MEGA_EQ_XT=1|2 overrides.
   uint32_t T = p.col_tiles > 1 ? 2 : 1;
-  const int xt = env_int("MEGA_EQ_XT");
+  const int xt = tune(kEqXt);
   if (xt == 1 || xt == 2) T = (uint32_t)xt;
   p.nmg = (a.U + 16 * MT - 1) / (16 * MT);
   p.nwk = 1;
@@ -446,7 +441,7 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp, hipStrea
   // waves per CU, e.g. one slot of a 256x32 cell: 30 us instead of 36), none
   // otherwise; MEGA_EQ_XKS=1|2|4 overrides.
   uint32_t ks = split_row(a.B) == 256 && units < 2048 ? 2 : 1;
-  const int xks = env_int("MEGA_EQ_XKS");
+  const int xks = tune(kEqXks);
   if (xks == 1 || xks == 2 || xks == 4) ks = (uint32_t)xks;
   const uint32_t* w = static_cast<const uint32_t*>(wp);
   const uint32_t n = (uint32_t)units;

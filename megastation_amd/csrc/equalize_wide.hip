@@ -1,0 +1,181 @@
+// FP32 equalize + demod for wide cells (up to 32 users, 128 or 256
+// antennas, e.g. 256 x 32): the antennas are split over the four waves of a
+// workgroup, each wave keeps its slice of W_g in registers for the whole
+// unit, and the complex products take three real MFMAs instead of four.
+//
+// Unit = (slot, group, chunk of TC column tiles of 16 (symbol, subcarrier)
+// columns). Wave w owns antennas 4*KS*w .. 4*KS*(w+1)-1 (KS steps of 4) and
+// computes, for every column tile of the unit, the partial products over its
+// antennas for all 16*MT users; the four partial tiles meet in LDS (double
+// buffered, one barrier per tile) and each wave adds up (in wave order),
+// demodulates and stores a quarter of the tile's outputs.
+//
+// Products (Gauss): with W = Wr + i Wi and y = yr + i yi, per 4-antenna step
+// and 16-user x 16-column tile
+//   K1 += (Wr + Wi) yr,   K2 += Wr (yi - yr),   K3 += Wi (yr + yi)
+// on v_mfma_f32_16x16x4_f32 (FP32 products, FP32 accumulation), and at the
+// end Re x = K1 - K3, Im x = K1 + K2. Three MFMAs per step where the
+// interleaved form (eq_core.h) takes four: the FP32 form at 256 x 32 is bound
+// by the matrix cores' issue rate as much as by HBM. Rounding: every term is
+// an FP32 product of sums of two FP32 values, so the error bound is that of
+// the four-product form with |W| and |y| replaced by |Wr| + |Wi| and
+// |yr| + |yi| (at most twice as large); parity is judged against the FP64
+// oracle like every FP32 form.
+//
+// Operand layout of the 16x16x4 instruction: lane l supplies A[l%16][l/16]
+// (user 16*mt + l%16, antenna 4*s + l/16 of the wave's slice) and
+// B[l/16][l%16] (that antenna, column l%16 of the tile); the result lane l
+// holds rows 4*(l/16) + r, column l%16.
+#include "eq_core.h"
+
+namespace mega {
+namespace detail {
+namespace {
+
+template <int MT, int KS, int Q>
+__global__ __launch_bounds__(256, 2) void equalize_wide(const EqParams p, uint32_t chunks,
+                                                        uint32_t tc, uint32_t units) {
+  // partial x tiles: [buffer][wave][16-user tile][re, im][row r][lane]
+  __shared__ float red[2][4][MT][2][4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i16 = lane & 15, q4 = lane >> 4;
+  const uint32_t unit = xcd_linear(blockIdx.x, gridDim.x);
+  if (unit >= units) return;  // uniform over the workgroup
+  const uint32_t c = unit % chunks;
+  const uint32_t rest = unit / chunks;
+  const uint32_t g = rest % p.G, slot = rest / p.G;
+  const uint32_t U = p.U, B = p.B, N = p.N;
+  const uint32_t k0 = g * p.sg;
+  const uint32_t nk = min(p.sg, N - k0);
+  const uint32_t ncol = nk * p.S;
+  const uint32_t t0 = c * tc;
+  const uint32_t t1 = min(t0 + tc, (ncol + 15) / 16);
+  if (t0 >= t1) return;  // a chunk past a partial group: uniform
+
+  // ---- this wave's W slice: Wr + Wi, Wr, Wi per (user tile, step) -----------
+  const float2* W = p.csi + slot * p.csi_slot + (uint64_t)g * U * B;
+  const uint32_t bw = 4 * KS * w;  // first antenna of the slice
+  float wsum[MT][KS], wre[MT][KS], wim[MT][KS];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const uint32_t u = 16 * m + i16;
+    const float2* wrow = W + min(u, U - 1);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const uint32_t b = bw + 4 * s + q4;
+      const float2 v = wrow[(uint64_t)min(b, B - 1) * U];
+      const bool ok = u < U && b < B;
+      wre[m][s] = ok ? v.x : 0.f;
+      wim[m][s] = ok ? v.y : 0.f;
+      wsum[m][s] = ok ? v.x + v.y : 0.f;
+    }
+  }
+
+  // ---- y of a column tile: this lane's column, the slice's antennas --------
+  // (columns past the group's end and antennas past B re-read valid
+  // addresses; their products are discarded or multiplied by W = 0)
+  auto load_y = [&](uint32_t t, float2 (&yd)[KS]) {
+    const uint32_t n = min(16 * t + i16, ncol - 1);
+    const uint32_t sym = n / nk;
+    const float2* yc = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
+#pragma unroll
+    for (int s = 0; s < KS; ++s) yd[s] = yc[(uint64_t)min(bw + 4 * s + q4, B - 1) * N];
+  };
+
+  float2 ycur[KS], ynext[KS];
+  load_y(t0, ynext);
+  int buf = 0;
+  for (uint32_t t = t0; t < t1; ++t) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s) ycur[s] = ynext[s];
+    if (t + 1 < t1) load_y(t + 1, ynext);  // in flight during this tile's MFMAs
+    __builtin_amdgcn_sched_barrier(0);
+    v4f k1[MT], k2[MT], k3[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) k1[m] = k2[m] = k3[m] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const float yr = ycur[s].x;
+      const float yd = ycur[s].y - ycur[s].x;
+      const float yp = ycur[s].x + ycur[s].y;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        k1[m] = mfma4(wsum[m][s], yr, k1[m]);
+        k2[m] = mfma4(wre[m][s], yd, k2[m]);
+        k3[m] = mfma4(wim[m][s], yp, k3[m]);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        red[buf][w][m][0][r][lane] = k1[m][r] - k3[m][r];
+        red[buf][w][m][1][r][lane] = k1[m][r] + k2[m][r];
+      }
+    __syncthreads();
+    // wave w stores outputs (m, r) = pi / 4, pi % 4 for pi = w, w + 4, ...
+    const EqCol col = eq_col<Q>(p, slot, g, t);
+#pragma unroll
+    for (int pi = w; pi < 4 * MT; pi += 4) {
+      const int m = pi / 4, r = pi % 4;
+      float re = red[buf][0][m][0][r][lane], im = red[buf][0][m][1][r][lane];
+#pragma unroll
+      for (int j = 1; j < 4; ++j) {
+        re += red[buf][j][m][0][r][lane];
+        im += red[buf][j][m][1][r][lane];
+      }
+      const uint32_t u = 16 * m + 4 * q4 + r;
+      if (col.valid && u < U) {
+        store_llrs<Q>(p, col.rows + u * p.ue_stride, col.ls, re, im);
+        if (col.xrow) col.xrow[(uint64_t)u * N] = make_float2(re, im);
+      }
+    }
+    buf ^= 1;
+  }
+}
+
+template <int MT, int KS>
+hipError_t launch_q(const EqParams& p, uint32_t chunks, uint32_t tc, uint32_t units,
+                    uint32_t Q, hipStream_t st) {
+  const dim3 grid(units), block(256);
+  switch (Q) {
+    case 2: hipLaunchKernelGGL((equalize_wide<MT, KS, 2>), grid, block, 0, st, p, chunks, tc, units); break;
+    case 4: hipLaunchKernelGGL((equalize_wide<MT, KS, 4>), grid, block, 0, st, p, chunks, tc, units); break;
+    case 6: hipLaunchKernelGGL((equalize_wide<MT, KS, 6>), grid, block, 0, st, p, chunks, tc, units); break;
+    case 8: hipLaunchKernelGGL((equalize_wide<MT, KS, 8>), grid, block, 0, st, p, chunks, tc, units); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+bool eq_wide_ok(uint32_t U, uint32_t B) { return U >= 1 && U <= 32 && B > 32 && B <= 256; }
+
+// Column tiles per workgroup: the fewest that still give every CU about four
+// workgroups (1024 in the grid), so each wave's W slice serves as many tiles
+// as the grid's size allows; tc_override (>= 1) replaces it for sweeps.
+hipError_t launch_equalize_wide(const EqParams& p0, uint32_t tc_override, hipStream_t st,
+                                const char** why) {
+  EqParams p = p0;
+  const uint32_t MT = p.U <= 16 ? 1 : 2;
+  const uint32_t KS = p.B <= 128 ? 8 : 16;
+  const uint64_t groups = (uint64_t)p.slots * p.G;
+  uint32_t tc = 1;
+  while (tc < p.col_tiles && groups * ((p.col_tiles + tc) / (tc + 1)) >= 1024) ++tc;
+  if (tc_override >= 1) tc = tc_override < p.col_tiles ? tc_override : p.col_tiles;
+  const uint32_t chunks = (p.col_tiles + tc - 1) / tc;
+  const uint64_t units = groups * chunks;
+  if (units > 0x7fffffffull) {
+    *why = "grid too large";
+    return hipErrorInvalidValue;
+  }
+  const uint32_t n = (uint32_t)units;
+  if (MT == 1) return KS == 8 ? launch_q<1, 8>(p, chunks, tc, n, p.Q, st)
+                              : launch_q<1, 16>(p, chunks, tc, n, p.Q, st);
+  return KS == 8 ? launch_q<2, 8>(p, chunks, tc, n, p.Q, st)
+                 : launch_q<2, 16>(p, chunks, tc, n, p.Q, st);
+}
+
+}  // namespace detail
+}  // namespace mega

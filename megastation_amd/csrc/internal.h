@@ -105,6 +105,17 @@ hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,
                                hipStream_t stream);
 
+// Tuning overrides (tuning.cpp): MEGA_* environment variables parsed once
+// (and on mega_tuning_reload), sanitized to each key's accepted range; 0 is
+// the launcher's own choice except for kDetPf (default 1).
+enum TuneKey {
+  kEqKs, kEqMode, kEqSch, kEqSmt, kEqTpw, kEqXmt, kEqXreg, kEqXt, kEqXks, kPrecAcc, kDetPf,
+  kEqWtc, kZfNth,
+  kTuneCount
+};
+int tune(TuneKey k);
+void tuning_reload();
+
 // Limits of the kernels (checked by the C ABI before launching).
 constexpr uint32_t kMaxUsers = 64;
 

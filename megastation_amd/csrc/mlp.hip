@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
 
 #include "mlp.h"
@@ -220,9 +221,11 @@ uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t)
 }  // namespace
 
 // K splits: enough workgroups to cover the CUs (>= 256) while every split
-// keeps at least 8 K steps; the partial sums live in a per-device workspace
-// that grows on demand (hipMalloc outside any capture; the GEMMs of the
-// network are not captured).
+// keeps at least 8 K steps. The partial sums live in a workspace owned by the
+// library per (device, stream): GEMMs queued on one stream run in order and
+// share it safely, GEMMs on different streams never share one. It grows on
+// demand with hipFree/hipMalloc, so a split GEMM is refused while its stream
+// is being captured into a graph.
 hipError_t mlp_gemm(const MlpGemm& g, hipStream_t st) {
   if (g.M == 0 || g.N == 0) return hipSuccess;
   const uint32_t tiles = ((g.N + TN - 1) / TN) * ((g.M + TM - 1) / TM);
@@ -233,27 +236,33 @@ hipError_t mlp_gemm(const MlpGemm& g, hipStream_t st) {
   splits = (g.K + kchunk - 1) / kchunk;
   float* part = nullptr;
   if (splits > 1) {
-    static std::mutex mu;
-    static void* ws[64] = {};
-    static size_t ws_bytes[64] = {};
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(st, &cap);
     if (e != hipSuccess) return e;
-    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (cap != hipStreamCaptureStatusNone) return hipErrorStreamCaptureUnsupported;
+    struct Ws {
+      void* p = nullptr;
+      size_t bytes = 0;
+    };
+    static std::mutex mu;
+    static std::map<std::pair<int, hipStream_t>, Ws> ws;
+    int dev = 0;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
     const size_t need = (size_t)splits * g.M * g.N * sizeof(float);
     std::lock_guard<std::mutex> lock(mu);
-    if (ws_bytes[dev] < need) {
-      if (ws[dev]) {
-        e = hipFree(ws[dev]);  // synchronizes: earlier GEMMs are done with it
+    Ws& w = ws[{dev, st}];
+    if (w.bytes < need) {
+      if (w.p) {
+        e = hipFree(w.p);  // synchronizes: earlier GEMMs are done with it
         if (e != hipSuccess) return e;
-        ws[dev] = nullptr;
-        ws_bytes[dev] = 0;
+        w = Ws{};
       }
-      e = hipMalloc(&ws[dev], need);
+      e = hipMalloc(&w.p, need);
       if (e != hipSuccess) return e;
-      ws_bytes[dev] = need;
+      w.bytes = need;
     }
-    part = static_cast<float*>(ws[dev]);
+    part = static_cast<float*>(w.p);
   }
   const dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
   hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, st, g.A, g.sai, g.sak, g.B, g.sbk, g.sbj,

@@ -130,8 +130,7 @@ hipError_t launch_precode(const PrecodeArgs& a, hipStream_t st, const char** why
   const dim3 grid((uint32_t)(G * nab)), block(256);
   // 32 antennas per wave (fewer registers, one more wave per SIMD at 32
   // users: 3% faster at 256x32); MEGA_PREC_ACC=8 selects 64
-  const char* e = std::getenv("MEGA_PREC_ACC");
-  const bool half = !(e && std::atoi(e) == 8);
+  const bool half = tune(kPrecAcc) != 8;
 #define MEGA_PREC(UM)                                                                       \
   if (half)                                                                                  \
     hipLaunchKernelGGL((precode_mfma<UM, 4>), grid, block, 0, st, a, (uint32_t)nab,          \

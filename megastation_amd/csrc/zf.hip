@@ -7,18 +7,21 @@ namespace mega {
 namespace detail {
 namespace {
 
+// Threads per group: one wave for up to 16 users, four waves otherwise; for
+// 17 to 32 users MEGA_ZF_NTH = 512 spreads a group over 8 waves (tuning
+// sweeps).
 template <int UT>
 constexpr int zf_threads() { return UT == 1 ? 64 : 256; }
 
-template <int UT, bool A_LDS>  // U <= 16 * UT
-__global__ __launch_bounds__(zf_threads<UT>()) void zf_mfma(
+template <int UT, bool A_LDS, int NTH>  // U <= 16 * UT
+__global__ __launch_bounds__(NTH) void zf_mfma(
     const float2* in, float2* out, int32_t* info, uint32_t U, uint32_t B) {
   __shared__ ZfShared<UT> sh;
   extern __shared__ float sDyn[];
   float2* sA = reinterpret_cast<float2*>(sDyn);
   float2* part = reinterpret_cast<float2*>(sDyn + (A_LDS ? 2 * (U + 1) * B : 0));
   const uint32_t g = blockIdx.x;
-  const int fail = zf_group<UT, zf_threads<UT>(), A_LDS>(
+  const int fail = zf_group<UT, NTH, A_LDS>(
       in + (uint64_t)g * U * B, U, B, sh, sA, part, out + (uint64_t)g * U * B, U, nullptr);
   if (threadIdx.x == 0 && info) info[g] = fail;
 }
@@ -27,24 +30,35 @@ __global__ __launch_bounds__(zf_threads<UT>()) void zf_mfma(
 // 160 KB a workgroup may use; A is staged (up to 16 users) only when that
 // holds. Wider groups read A from global memory so that several groups fit
 // a CU at once.
-template <int UT>
-hipError_t launch_zf_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
-                        float2* out, int32_t* info, hipStream_t st) {
-  constexpr int NTH = zf_threads<UT>();
+template <int UT, int NTH>
+hipError_t launch_zf_nth(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                         float2* out, int32_t* info, hipStream_t st) {
   const size_t with_a = zf_dyn_lds<UT, NTH>(U, B, true);
   const bool a_lds = UT == 1 && sizeof(ZfShared<UT>) + with_a <= 120 * 1024;
   const size_t lds = a_lds ? with_a : zf_dyn_lds<UT, NTH>(U, B, false);
-  const void* fn = a_lds ? reinterpret_cast<const void*>(&zf_mfma<UT, true>)
-                         : reinterpret_cast<const void*>(&zf_mfma<UT, false>);
+  const void* fn = a_lds ? reinterpret_cast<const void*>(&zf_mfma<UT, true, NTH>)
+                         : reinterpret_cast<const void*>(&zf_mfma<UT, false, NTH>);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   if (a_lds)
-    hipLaunchKernelGGL((zf_mfma<UT, true>), dim3(batch), dim3(NTH), lds, st, in, out, info, U, B);
+    hipLaunchKernelGGL((zf_mfma<UT, true, NTH>), dim3(batch), dim3(NTH), lds, st, in, out, info,
+                       U, B);
   else
-    hipLaunchKernelGGL((zf_mfma<UT, false>), dim3(batch), dim3(NTH), lds, st, in, out, info, U, B);
+    hipLaunchKernelGGL((zf_mfma<UT, false, NTH>), dim3(batch), dim3(NTH), lds, st, in, out,
+                       info, U, B);
   return hipGetLastError();
+}
+
+template <int UT>
+hipError_t launch_zf_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                        float2* out, int32_t* info, hipStream_t st) {
+  if constexpr (UT == 2) {
+    const int nth = tune(kZfNth);
+    if (nth == 512) return launch_zf_nth<2, 512>(U, B, batch, in, out, info, st);
+  }
+  return launch_zf_nth<UT, zf_threads<UT>()>(U, B, batch, in, out, info, st);
 }
 
 // P_g = W_g^T * (1 / max|W_g|) (beam_utils.cuh:40-80): U x B column-major in,

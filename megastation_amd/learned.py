@@ -201,6 +201,82 @@ class WarmStartNet:
         self.sync_bf16()
 
 
+# ---------------------------------------------------------------------------
+# Checkpoints: the FP32 master weights and biases of a WarmStartNet in a
+# numpy .npz (no pickled objects) with the network's shape, what it was
+# trained on, and a SHA-256 over every array's dtype, shape and bytes in a
+# fixed order; read_checkpoint recomputes it and refuses a file that does not
+# match.
+# ---------------------------------------------------------------------------
+
+CKPT_ARRAYS = ("w0", "w1", "w2", "b0", "b1", "b2")
+
+
+class CheckpointError(ValueError):
+    pass
+
+
+def _digest(arrays: dict, meta: str) -> str:
+    import hashlib
+    h = hashlib.sha256()
+    h.update(meta.encode())
+    for k in CKPT_ARRAYS:
+        a = np.ascontiguousarray(arrays[k])
+        h.update(f"{k}:{a.dtype.str}:{a.shape}".encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def write_checkpoint(path: str, arrays: dict, info: dict) -> str:
+    """Writes weights (w0..w2, b0..b2: float32) and info (JSON-able dict) to
+    path (.npz); returns the SHA-256 stored with them."""
+    import json
+    meta = json.dumps(info, sort_keys=True)
+    arrs = {k: np.ascontiguousarray(arrays[k], dtype=np.float32) for k in CKPT_ARRAYS}
+    digest = _digest(arrs, meta)
+    with open(path, "wb") as fh:
+        np.savez_compressed(fh, meta=np.array(meta), sha256=np.array(digest), **arrs)
+    return digest
+
+
+def read_checkpoint(path: str) -> tuple[dict, dict]:
+    """(arrays, info) of a checkpoint, after checking its SHA-256 (raises
+    CheckpointError on a mismatch or a missing array). CPU only."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        missing = [k for k in CKPT_ARRAYS + ("meta", "sha256") if k not in z.files]
+        if missing:
+            raise CheckpointError(f"{path}: missing {missing}")
+        arrs = {k: z[k] for k in CKPT_ARRAYS}
+        meta = str(z["meta"])
+        stored = str(z["sha256"])
+    if _digest(arrs, meta) != stored:
+        raise CheckpointError(f"{path}: SHA-256 mismatch (file corrupted or altered)")
+    info = json.loads(meta)
+    dims = info.get("dims")
+    if dims is None or [arrs[f"w{l}"].shape for l in range(3)] != \
+            [(dims[l], dims[l + 1]) for l in range(3)]:
+        raise CheckpointError(f"{path}: weight shapes do not match dims {dims}")
+    return arrs, info
+
+
+def save(net: "WarmStartNet", path: str, info: dict | None = None) -> str:
+    """Checkpoint of net (FP32 master weights) with its shape and `info`."""
+    d = net.state_dict()
+    meta = dict(info or {}, U=net.U, dims=net.dims)
+    return write_checkpoint(path, d, meta)
+
+
+def load(path: str, seed: int = 0) -> tuple["WarmStartNet", dict]:
+    """A WarmStartNet with the checkpoint's weights (checksum verified)."""
+    arrs, info = read_checkpoint(path)
+    net = WarmStartNet(int(info["U"]), hidden=int(info["dims"][1]), seed=seed)
+    if net.dims != list(info["dims"]):
+        raise CheckpointError(f"{path}: dims {info['dims']} do not fit U={info['U']}")
+    net.load_state_dict(arrs)
+    return net, info
+
+
 class GpuChannel:
     """synth.TdlJakes evaluated on the GPU: slot(t) -> A = H^H [G][B][U]
     complex64 on cuda:0 (same parameters and random draws as the CPU class)."""
@@ -297,9 +373,11 @@ def training_pairs(chan: GpuChannel, t0: int, n_pairs: int, U: int, B: int):
 
 
 def train(net: WarmStartNet, chans, samples: int = 200_000, batch_pairs: int = 4,
-          lr: float = 1e-3, log=None):
+          lr: float = 1e-3, log=None, lr_at=None):
     """Trains on consecutive slot pairs of the given channels (round robin)
-    until `samples` group samples have been seen. Returns the loss history."""
+    until `samples` group samples have been seen. lr_at(seen) (optional)
+    gives the learning rate as a function of the samples seen so far.
+    Returns the loss history [(samples seen, loss)]."""
     import torch
     U, B = net.U, chans[0].ch.B
     G = chans[0].ch.G
@@ -318,7 +396,7 @@ def train(net: WarmStartNet, chans, samples: int = 200_000, batch_pairs: int = 4
         out = net.forward(x, keep=True)
         d = net.mse_grad(out, y)
         net.backward(d)
-        net.adam(lr=lr)
+        net.adam(lr=lr if lr_at is None else lr_at(seen))
         seen += x.shape[0]
         if len(losses) % 10 == 0 or seen >= samples:
             loss = float(0.5 * ((out - y) ** 2).sum(dim=1).mean())

@@ -253,6 +253,28 @@ def ldpc_uplink_layout(N: int, Q: int, code_rate: float = 1 / 3,
 # slots
 # --------------------------------------------------------------------------
 
+# Uneven batch of cells: antenna x user, subcarriers, group size and QAM
+# order drawn per cell from these sets (seeded), 13 data symbols, i.i.d.
+# CN(0,1) channels at 20 dB SNR.
+BATCH_SHAPES = [(64, 8), (64, 16), (128, 16), (128, 32), (256, 64)]   # (B, U)
+BATCH_N = [624, 1200, 3276]
+BATCH_SG = [12, 16]
+BATCH_Q = [4, 6, 8]
+
+
+def uneven_cells(n_cells: int = 40, seed: int = 7) -> list[dict]:
+    """Cell configurations of the uneven batch workload (dicts with U, B, N,
+    sg, S, slots, Q, snr, channel)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    out = []
+    for _ in range(n_cells):
+        B, U = BATCH_SHAPES[rng.integers(len(BATCH_SHAPES))]
+        out.append(dict(U=U, B=B, N=int(BATCH_N[rng.integers(3)]),
+                        sg=int(BATCH_SG[rng.integers(2)]), S=13, slots=1,
+                        Q=int(BATCH_Q[rng.integers(3)]), snr=20.0, channel="iid"))
+    return out
+
+
 @dataclass
 class Slot:
     csi: np.ndarray    # complex64 [G][B][U]  (A = H^H)
@@ -282,6 +304,42 @@ def make_slot(rng, U: int, B: int, N: int, sg: int, S: int, Q: int,
     y = np.einsum("kbu,suk->sbk", Hk, x, optimize=True)
     y = (y + _cn(rng, y.shape, sigma2)).astype(np.complex64)
     return Slot(csi=np.conj(H).astype(np.complex64), y=y, x=x, bits=bits, H=H)
+
+
+def index_bits(idx: np.ndarray, Q: int) -> np.ndarray:
+    """Symbol indices -> bits[..., Q], first stream bit (the index MSB)
+    first: the order of a symbol's Q LLRs."""
+    sh = np.arange(Q - 1, -1, -1)
+    return ((np.asarray(idx)[..., None] >> sh) & 1).astype(np.uint8)
+
+
+# Per-level thresholds of the max-log demod in units of the LLR scale 100
+# (src/operators/modulation/demodulation.h:35-67): 200/sqrt(10);
+# 400/sqrt(42), 200/sqrt(42); 800/sqrt(170), 400/sqrt(170), 200/sqrt(170).
+_DEMOD_T = {4: (2 / math.sqrt(10),), 6: (4 / math.sqrt(42), 2 / math.sqrt(42)),
+            8: (8 / math.sqrt(170), 4 / math.sqrt(170), 2 / math.sqrt(170))}
+
+
+def hard_bits_xhat(xhat: np.ndarray, Q: int) -> np.ndarray:
+    """Hard decisions of the max-log demod taken in float64 straight from
+    x_hat (no binary16 rounding): bit j = 1 iff LLR j < 0, with
+    l0 = re, l1 = im, l(2i+2) = t_i - |l(2i)|, l(2i+3) = t_i - |l(2i+1)|
+    (QPSK: l = -re, -im). Returns uint8 [..., Q]."""
+    x = np.asarray(xhat).astype(np.complex128)
+    re, im = x.real, x.imag
+    if Q == 2:
+        ls = [-re, -im]
+    else:
+        ls = [re, im]
+        for t in _DEMOD_T[Q]:
+            ls += [t - np.abs(ls[-2]), t - np.abs(ls[-1])]
+    return np.stack([(l < 0) for l in ls], axis=-1).astype(np.uint8)
+
+
+def hard_bits_llr(llr_u16: np.ndarray) -> np.ndarray:
+    """Hard decisions from binary16 LLR bit patterns: the sign bit
+    (LLR > 0 means bit 0)."""
+    return (np.asarray(llr_u16).astype(np.uint16) >> 15).astype(np.uint8)
 
 
 def evm_db(xhat: np.ndarray, x: np.ndarray) -> float:

@@ -27,3 +27,19 @@ def golden():
     path = os.path.join(ROOT, "tests", "golden", "reference_fixtures.npz")
     with np.load(path, allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
+
+
+@pytest.fixture
+def tuning(monkeypatch):
+    """Sets MEGA_* tuning overrides for one test, tuning(MEGA_EQ_MODE=1, ...):
+    the library parses them once, so it is told to re-read them now and
+    again after the test has restored the environment."""
+    import megastation_amd as m
+
+    def set_(**kv):
+        for k, v in kv.items():
+            monkeypatch.setenv(k, str(v))
+        m.tuning_reload()
+    yield set_
+    monkeypatch.undo()
+    m.tuning_reload()

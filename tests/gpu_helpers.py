@@ -35,6 +35,23 @@ def host_c64(t) -> np.ndarray:
     return t.cpu().numpy()
 
 
+def rel_fro(a, b) -> float:
+    """||a - b||_F / ||b||_F. Complex inputs are compared as complex128 (real
+    and imaginary parts both count); real inputs as float64."""
+    a, b = np.asarray(a), np.asarray(b)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a, b = a.astype(dt), b.astype(dt)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def assert_imag_counts(ref, floor: float = 0.1) -> None:
+    """Guard for a complex check: for the test's own reference data the
+    conjugate must be far from it under rel_fro, so a comparison that
+    dropped the imaginary parts could not pass."""
+    assert np.iscomplexobj(ref), "complex check on real data"
+    assert rel_fro(np.conj(ref), ref) > floor, "reference has no imaginary content"
+
+
 def gpu_equalize_slots(csi, y, U, B, N, sg, Q, lay, *, slots, S, want_xhat=True,
                        fill=SENTINEL):
     """csi complex64 [slots][G][B][U], y [slots][S][B][N]. Returns (llr uint16

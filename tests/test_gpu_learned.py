@@ -13,7 +13,7 @@ import megastation_amd as m
 from megastation_amd import learned, synth
 from oracle import mlp
 
-from gpu_helpers import dev_c64, host_c64, torch
+from gpu_helpers import assert_imag_counts, dev_c64, host_c64, rel_fro, torch
 
 pytestmark = pytest.mark.gpu
 
@@ -23,10 +23,7 @@ def _bf(t_np):
     return t.from_numpy(np.asarray(t_np, dtype=np.float32)).cuda().to(t.bfloat16)
 
 
-def _rel(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+_rel = rel_fro  # complex arrays compare as complex128
 
 
 @pytest.mark.parametrize("M,N,K", [(70, 45, 33), (64, 64, 64), (5, 130, 300)])
@@ -103,8 +100,10 @@ def test_gram_inv_matches_fp64(U, B):
     Gm, X = learned.gram_inv(dev_c64(a), U, B, G)
     Gn = host_c64(Gm).astype(np.complex128)
     ref = _gram64(a, G, B, U)
+    assert_imag_counts(ref)
     assert _rel(Gn, ref) < 1e-6
     Xn = host_c64(X).astype(np.complex128)
+    assert_imag_counts(np.linalg.inv(ref))
     assert _rel(Xn, np.linalg.inv(ref)) < 1e-4
     Gonly, none = learned.gram_inv(dev_c64(a), U, B, G, want_x=False)
     assert none is None and np.array_equal(host_c64(Gonly), host_c64(Gm))
@@ -124,6 +123,7 @@ def test_refine_first_order_newton_schulz_and_residual_check(U, B):
     # no correction, no iterations: the first-order update
     Xfo, res, flag = learned.refine(G1, G0, X0, U, iters=0)
     fo = x0 - x0 @ (g1 - g0) @ x0
+    assert_imag_counts(fo)
     assert _rel(host_c64(Xfo), fo) < 1e-4
     e_fo = np.linalg.norm(fo @ g1 - np.eye(U), axis=(1, 2))
     assert np.allclose(res.cpu().numpy(), e_fo, rtol=1e-3)
@@ -172,26 +172,53 @@ def test_finish_fallback_and_apply(U, B):
     A = a.reshape(G, B, U).astype(np.complex128)
     xn = host_c64(X).astype(np.complex128)
     ref = np.einsum("guv,gbv->gbu", xn, A).reshape(G, B * U)
+    assert_imag_counts(ref[~f])
     assert _rel(w[~f], ref[~f]) < 1e-5
     assert not info.cpu().numpy().any()
 
 
+def test_checkpoint_loads_into_the_same_network(tmp_path):
+    """save -> load gives a network whose forward pass is bit-identical."""
+    print("case: learned checkpoint", flush=True)
+    t = torch()
+    U = 4
+    net = learned.WarmStartNet(U, hidden=64, seed=3)
+    x = _bf(np.random.default_rng(1).standard_normal((5, learned.n_features(U))))
+    ref = net.forward(x).cpu().numpy()
+    p = str(tmp_path / "n.npz")
+    learned.save(net, p, {"note": "test"})
+    net2, info = learned.load(p, seed=9)
+    assert info["note"] == "test" and info["U"] == U
+    assert np.array_equal(net2.forward(x).cpu().numpy(), ref)
+    t.cuda.synchronize()
+
+
 def test_trained_network_reduces_fallbacks_on_held_out_sequence():
-    """Train on two generator sequences (200k group samples, the network's
-    own kernels), then run the learned ZF over 40 held-out slots of a third
-    sequence with and without the network: the network must cut both the
-    median residual |X G - I|_F after two Newton-Schulz steps and the share
-    of groups that fall back to the plain ZF; W of every group stays within
-    the residual check's bound of the plain ZF's W."""
+    """The network trained on two generator sequences (the committed
+    checkpoint of megastation_amd.train_learned: 200k slot pairs, the
+    network's own kernels, loaded with its SHA-256 checked; trained here on
+    200k group samples if the file is absent), run in the learned ZF over 40
+    held-out slots of a third sequence, with and without the network: the
+    network must cut both the median residual |X G - I|_F after two
+    Newton-Schulz steps and the share of groups that fall back to the plain
+    ZF; W of every group stays within the residual check's bound of the
+    plain ZF's W."""
+    import os
+    from megastation_amd import train_learned
     print("case: learned gain", flush=True)
     t = torch()
     U, B, N, sg = 32, 128, 3276, 12
-    chans = [learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=s)) for s in (11, 12)]
     test = learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=99))
-    net = learned.WarmStartNet(U, seed=0)
-    hist = learned.train(net, chans, samples=200_000)
-    assert hist[-1][1] < 0.7 * hist[0][1]
-    G = chans[0].ch.G
+    if os.path.exists(train_learned.DEFAULT_OUT):
+        net, info = learned.load(train_learned.DEFAULT_OUT)
+        print(f"loaded {train_learned.DEFAULT_OUT}: {info}", flush=True)
+        assert info["train_slot_pairs"] >= 200_000 and info["loss_last"] < 0.7 * info["loss_first"]
+    else:
+        chans = [learned.GpuChannel(synth.TdlJakes(B, U, N, sg, seed=s)) for s in (11, 12)]
+        net = learned.WarmStartNet(U, seed=0)
+        hist = learned.train(net, chans, samples=200_000)
+        assert hist[-1][1] < 0.7 * hist[0][1]
+    G = test.ch.G
     stats = {}
     for label, nx in (("without", None), ("with", net)):
         z = learned.LearnedZF(U, B, G, nx)

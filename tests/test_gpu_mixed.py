@@ -179,13 +179,13 @@ def test_detect_mixed_matches_fp64_oracle(U, B, N, sg, Q, channel, snr, layout, 
 
 @pytest.mark.parametrize("U,B,N,ks", [(32, 256, 600, 1), (32, 256, 600, 2), (32, 256, 600, 4),
                                       (64, 128, 300, 2), (16, 256, 250, 4)])
-def test_mixed_equalize_antenna_split(monkeypatch, U, B, N, ks):
+def test_mixed_equalize_antenna_split(tuning, U, B, N, ks):
     """The streamed mixed equalize with the antennas of a unit split over
     ks waves of a workgroup (partial sums met in LDS), every split the
     launcher can take, forced by MEGA_EQ_XKS; a split changes the FP32
     summation order only, so each is held to the FP64 oracle's bar
     (partial last group included: N % sg != 0)."""
-    monkeypatch.setenv("MEGA_EQ_XKS", str(ks))
+    tuning(MEGA_EQ_XKS=ks)
     print(f"case: mixed equalize antenna split ks={ks} U={U} B={B} N={N}", flush=True)
     sg, Q, S, slots = 12, 6, 2, 1
     sl = make_slots(U + B + N + ks, slots, U, B, N, sg, S, Q, 25.0, "iid")

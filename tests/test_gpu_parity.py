@@ -144,16 +144,20 @@ EQ_CASES = [
     (32, 128, 3276, 16, 8, "standalone"),   # partial last group (204.75 groups)
     (64, 128, 600, 12, 4, "ldpc"),
     (3, 5, 37, 7, 6, "standalone"),
+    (20, 200, 500, 12, 6, "ldpc"),          # wide form: 20 users, 200 antennas
+    (32, 96, 300, 7, 4, "standalone"),      # wide form: a part-filled antenna slice
 ]
 
 
 # MEGA_EQ_MODE: 0 = the launcher's choice, 1 = W streamed from memory,
-# 2 = W held in registers (antennas split over waves when B is wide)
-@pytest.mark.parametrize("mode", ["0", "1", "2"])
+# 2 = W held in registers (antennas split over waves when B is wide),
+# 3 = the wide form (antennas split over a workgroup's waves, three-product
+# MFMAs) where it applies (up to 32 users, 33 to 256 antennas)
+@pytest.mark.parametrize("mode", ["0", "1", "2", "3"])
 @pytest.mark.parametrize("U,B,N,sg,Q,layout", EQ_CASES)
-def test_equalize_matches_oracle(U, B, N, sg, Q, layout, mode, monkeypatch):
+def test_equalize_matches_oracle(U, B, N, sg, Q, layout, mode, tuning):
     print(f"case: equalize U={U} B={B} N={N} sg={sg} Q={Q} {layout} mode={mode}", flush=True)
-    monkeypatch.setenv("MEGA_EQ_MODE", mode)
+    tuning(MEGA_EQ_MODE=mode)
     S, slots = 2, 1
     sl = make_slots(U * 7 + N, slots, U, B, N, sg, S, Q)
     lay = synth.ldpc_uplink_layout(N, Q) if layout == "ldpc" else synth.standalone_layout(N, Q)
@@ -165,6 +169,32 @@ def test_equalize_matches_oracle(U, B, N, sg, Q, layout, mode, monkeypatch):
     for s in range(S):
         llr64, x64 = oracle_symbol_f64(w[0], y[0, s], U, B, N, sg, Q, lay)
         check_symbol(llr[0, s], xh[0, s], llr64, x64, N, sg, Q, lay)
+
+
+@pytest.mark.parametrize("U,B,N,sg,Q", [(32, 256, 3276, 12, 6), (24, 128, 610, 16, 4)])
+def test_equalize_wide_chunking_is_bit_identical(U, B, N, sg, Q, tuning):
+    """The wide form over two slots and three symbols: against the FP64
+    oracle, and bit-identical LLRs and x_hat whatever the number of column
+    tiles per workgroup (a tile's arithmetic does not depend on the chunk
+    it is in)."""
+    print(f"case: equalize wide chunking U={U} B={B} N={N} sg={sg} Q={Q}", flush=True)
+    S, slots = 3, 2
+    sl = make_slots(U + B + N, slots, U, B, N, sg, S, Q, 25.0)
+    lay = synth.ldpc_uplink_layout(N, Q)
+    G = (N + sg - 1) // sg
+    w = np.stack([oracle.zf(s_.csi, U, B, G, "f64")[0].reshape(G, B, U) for s_ in sl])
+    y = np.stack([s_.y for s_ in sl])
+    outs = []
+    for tc in (0, 1, 4, 100):
+        tuning(MEGA_EQ_MODE=3, MEGA_EQ_WTC=tc)
+        outs.append(gpu_equalize_slots(w, y, U, B, N, sg, Q, lay, slots=slots, S=S))
+    for llr, xh in outs[1:]:
+        assert np.array_equal(llr, outs[0][0]) and np.array_equal(xh, outs[0][1])
+    llr, xh = outs[0]
+    for s_ in range(slots):
+        for t_ in range(S):
+            llr64, x64 = oracle_symbol_f64(w[s_], y[s_, t_], U, B, N, sg, Q, lay)
+            check_symbol(llr[s_, t_], xh[s_, t_], llr64, x64, N, sg, Q, lay)
 
 
 def test_equalize_slots_equals_per_symbol_calls():
@@ -284,12 +314,12 @@ def _detect(csi, y, U, B, N, sg, Q, lay, slots, S, want_w=True, want_x=True):
                                         (16, 128, 600, 16, 6), (12, 192, 300, 12, 2),
                                         (16, 256, 240, 16, 4), (5, 24, 100, 7, 6),
                                         (32, 128, 480, 16, 6)])
-def test_detect_slots_equals_zf_then_equalize(U, B, N, sg, Q, monkeypatch):
+def test_detect_slots_equals_zf_then_equalize(U, B, N, sg, Q, tuning):
     """Fused kernel == ZF kernel followed by the equalize kernel with W in
     registers, bit for bit (same arithmetic and summation order), and x_hat /
     EVM against the FP64 oracle."""
     print(f"case: detect U={U} B={B} N={N} sg={sg} Q={Q}", flush=True)
-    monkeypatch.setenv("MEGA_EQ_MODE", "2")
+    tuning(MEGA_EQ_MODE=2)
     S, slots = 3, 2
     sl = make_slots(U * 31 + B, slots, U, B, N, sg, S, Q, 25.0)
     lay = synth.standalone_layout(N, Q)
