@@ -113,7 +113,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c3_256x32",
                     choices=sorted(WORKLOADS) + ["c4_40cells", "c5_learned",
-                                                 "d3_precode_256x32", "u3_fft_256x32"])
+                                                 "d3_precode_256x32", "u3_fft_256x32",
+                                                 "l3_ldpc_256x32"])
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -139,6 +140,8 @@ def main():
         return main_precode(args)
     if args.workload == "u3_fft_256x32":
         return main_fft(args)
+    if args.workload == "l3_ldpc_256x32":
+        return main_ldpc(args)
 
     if args.copy_calibration:
         import torch
@@ -902,6 +905,88 @@ def main_fft(args):
                                   "cores": 1, "kind": "port",
                                   "sample": "1 of 13 data symbols (256 antennas), median of 3, "
                                             "numpy FFT (the oracle)"}
+    print(json.dumps(result), flush=True)
+
+
+def main_ldpc(args):
+    """LDPC decoding of the code blocks of one 256 x 32 slot (mega_ldpc_decode):
+    32 users x 13 symbols, one rate-1/3 BG1 code block each (Zc = 288, 46
+    parity rows: the LDPC layout the equalizer writes for 3276 subcarriers of
+    64-QAM), binary16 LLRs in 128-byte padded rows, 8 layered min-sum
+    iterations. LLRs: the all-zero codeword (a codeword of every linear code;
+    the decoder's work does not depend on the bits: fixed iterations, no
+    early stop) through BPSK + AWGN at 1 dB. Four rotating buffer sets. The
+    CPU baseline is the oracle's single-threaded C decoder on a sample of
+    the code blocks."""
+    import torch
+    import megastation_amd as m
+    from megastation_amd import synth
+    U, S, bg, zc, mb, iters = 32, 13, 1, 288, 46, 8
+    lay = synth.ldpc_uplink_layout(3276, 6)
+    assert lay.zeros == 2 * zc and lay.cb_size == (20 + mb) * zc
+    kb = 22
+    n_cb = U * S
+    nv = (kb + mb) * zc
+    words = (kb * zc + 31) // 32
+    rng = np.random.default_rng(3)
+    sigma = math.sqrt(0.5 / 10 ** (1.0 / 10))
+    llr = np.zeros((n_cb, lay.cb_stride), np.float16)
+    llr[:, :nv] = (2 * (1.0 + sigma * rng.standard_normal((n_cb, nv))) /
+                   sigma ** 2).astype(np.float16)
+    llr[:, :2 * zc] = 0
+    dev = torch.device("cuda:0")
+    NR = 4
+    L = [torch.from_numpy(llr.reshape(-1)).to(dev) for _ in range(NR)]
+    O = [torch.zeros(n_cb * words, dtype=torch.int32, device=dev) for _ in range(NR)]
+    stream = torch.cuda.current_stream()
+    kw = dict(base_graph=bg, lifting_size=zc, parity_nodes=mb, max_iterations=iters,
+              num_cb=n_cb, llr_stride=lay.cb_stride, out_stride=words, stream=stream)
+
+    def step(j):
+        m.ldpc_decode(L[j % NR], O[j % NR], **kw)
+    for j in range(args.warmup):
+        step(j)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for j in range(args.steps):
+        step(j)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    dec = np.unpackbits(O[0].cpu().numpy().view(np.uint8).reshape(n_cb, -1), axis=1,
+                        bitorder="little")[:, :kb * zc]
+    ber_ch = float(np.mean(llr[:, 2 * zc:kb * zc].astype(np.float32) < 0))
+    io = n_cb * (nv * 2 + words * 4)
+    result = {
+        "metric": "decoded information bits/s (LDPC, layered min-sum), 1 GPU",
+        "value": n_cb * kb * zc / (ms * 1e-3), "unit": "bits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (binary16 LLRs)", "data": "synthetic",
+        "config": {"workload": "l3_ldpc_256x32", "code_blocks": n_cb, "base_graph": bg,
+                   "lifting_size": zc, "parity_rows": mb, "iterations": iters,
+                   "snr_db_bpsk": 1.0, "parallelism": "1 GPU, one workgroup per code block"},
+        "roofline": {"bound": "hbm", "kernel": "ldpc_decode",
+                     "achieved": io / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": io / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": io, "avg_ms": ms},
+        "decode": {"code_blocks_per_s": n_cb / (ms * 1e-3),
+                   "channel_ber": ber_ch, "decoded_ber": float(np.mean(dec != 0))},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import ldpc as oldpc
+        sample = 16
+        l32 = llr[:sample, :nv].astype(np.float32)
+        ts = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            oldpc.decode(l32, bg, zc, mb, iters, 0.8125)
+            ts.append(time.perf_counter() - c0)
+        result["cpu_baseline"] = {"value": sample * kb * zc / statistics.median(ts),
+                                  "unit": "bits/s", "cores": 1, "kind": "port",
+                                  "sample": f"{sample} of {n_cb} code blocks, median of 3, "
+                                            "oracle C layered min-sum (single thread)"}
     print(json.dumps(result), flush=True)
 
 

@@ -458,6 +458,36 @@ int mega_mlp_adam(float *w, void *wb, const float *g, float *m, float *v, uint64
                   float beta1, float beta2, float eps, uint32_t step, mega_stream_t stream);
 int mega_mlp_to_bf16(const float *x, void *y, uint64_t n, mega_stream_t stream);
 
+/* ---- LDPC consumer: decode the equalizer's code blocks ---------------------
+ * Replaces LDPCDecoder(llr_type, LDPCConfig(mb, bg, Zc, max_iter)) and its
+ * operator()(tLLRs, tOut, stream) (src/operators/ldpc/decoder.h:40-86,
+ * config.h:27-73), which hand the decoding to cuPHY. Per code block:
+ * (Kb + mb) * Zc LLRs (Kb = 22 for base graph 1, 10 for base graph 2) with
+ * the 2 * Zc punctured systematic bits at zero, LLR > 0 meaning bit 0 -- the
+ * rows mega_equalize writes with zeros = 2 * Zc and cb_size = (Kb - 2 + mb) *
+ * Zc; out: the Kb * Zc information bits, 32 per word, LSB first. Decoding is
+ * layered normalized min-sum over the base graph of 3GPP TS 38.212 for the
+ * lifting size's set, max_iterations passes over the mb block rows (no
+ * early stop), check messages scaled by `normalization` (0 selects 0.8125).
+ * cuPHY's own schedule and scale are not public, so the decoded bits equal
+ * the reference's on codewords that both decode, not bit for bit in
+ * general. */
+typedef struct {
+  uint32_t base_graph;     /* 1 or 2 */
+  uint32_t lifting_size;   /* Zc, one of TS 38.212 Table 5.3.2-1 (2 .. 384) */
+  uint32_t parity_nodes;   /* mb: 4 .. 46 (base graph 1) or 4 .. 42 (2) */
+  uint32_t max_iterations; /* 1 .. 100 */
+  float normalization;     /* in (0, 1]; 0 = 0.8125 */
+  int32_t llr_bits;        /* 16: binary16 LLRs, 32: float */
+  const void *llr;
+  uint64_t llr_stride;     /* elements between code blocks, >= (Kb + mb) * Zc */
+  uint32_t num_cb;
+  uint32_t *out;
+  uint64_t out_stride;     /* 32-bit words between code blocks, >= ceil(Kb * Zc / 32) */
+} mega_ldpc_desc;
+
+int mega_ldpc_decode(const mega_ldpc_desc *desc, mega_stream_t stream);
+
 /* ---- host demodulation ----------------------------------------------------
  * Replaces the host side of demodQPSK / demod16QAM / demod64QAM /
  * demod256QAM (demodulation.h:30-67, __device__ __host__): n complex64

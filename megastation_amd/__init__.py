@@ -99,6 +99,16 @@ class PrecodeDesc(ctypes.Structure):
                 ("bsvec", ctypes.c_void_p), ("bs_sym_stride", ctypes.c_uint64)]
 
 
+class LdpcDesc(ctypes.Structure):
+    """mega_ldpc_desc (include/megadetect.h)."""
+    _fields_ = [("base_graph", ctypes.c_uint32), ("lifting_size", ctypes.c_uint32),
+                ("parity_nodes", ctypes.c_uint32), ("max_iterations", ctypes.c_uint32),
+                ("normalization", ctypes.c_float), ("llr_bits", ctypes.c_int32),
+                ("llr", ctypes.c_void_p), ("llr_stride", ctypes.c_uint64),
+                ("num_cb", ctypes.c_uint32), ("out", ctypes.c_void_p),
+                ("out_stride", ctypes.c_uint64)]
+
+
 class FftDesc(ctypes.Structure):
     """mega_fft_desc (include/megadetect.h)."""
     _fields_ = [("ofdm_start", ctypes.c_uint32), ("ofdm_ca", ctypes.c_uint32),
@@ -116,6 +126,12 @@ def lib() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run `python -m megastation_amd.build`"
                            " (the detection path has no fallback)")
+    # torch (the device memory and streams the library works on) ships its
+    # own HIP and HSA runtimes under the same sonames as the system's. Loading
+    # torch first makes the dynamic linker bind this library to torch's
+    # copies, so the process holds one HIP runtime; loading this library
+    # first would bind torch to the system copies instead.
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     P, u8, u32, u64, i32 = (ctypes.c_void_p, ctypes.c_uint8, ctypes.c_uint32,
                             ctypes.c_uint64, ctypes.c_int)
@@ -143,6 +159,7 @@ def lib() -> ctypes.CDLL:
         "mega_pilot_sign": ([u32, P], i32),
         "mega_downlink_ifft": ([ctypes.POINTER(FftDesc), P], i32),
         "mega_precode_syms": ([ctypes.POINTER(PrecodeDesc), P], i32),
+        "mega_ldpc_decode": ([ctypes.POINTER(LdpcDesc), P], i32),
         "mega_batch_create": ([ctypes.POINTER(CellDesc), u32, ctypes.c_char_p,
                                ctypes.POINTER(P)], i32),
         "mega_batch_run": ([P, P], i32),
@@ -357,6 +374,17 @@ def downlink_ifft(x, iq, *, ofdm_ca, bs_num, num_syms: int = 1, in_sym_stride: i
                   out_sym_stride)
     d.in_, d.out = _ptr(x).value, _ptr(iq).value
     _check(lib().mega_downlink_ifft(ctypes.byref(d), _stream(stream)))
+
+
+def ldpc_decode(llr, out, *, base_graph, lifting_size, parity_nodes, max_iterations, num_cb,
+                llr_stride, out_stride, normalization: float = 0.0, stream=None):
+    """LDPC decoding of num_cb code blocks (mega_ldpc_decode): llr a float16 or
+    float32 device tensor, out an int32 tensor of bit words."""
+    import torch
+    bits = 16 if llr.dtype in (torch.float16, torch.int16) else 32
+    d = LdpcDesc(base_graph, lifting_size, parity_nodes, max_iterations, normalization, bits,
+                 _ptr(llr).value, llr_stride, num_cb, _ptr(out).value, out_stride)
+    _check(lib().mega_ldpc_decode(ctypes.byref(d), _stream(stream)))
 
 
 def pilot_sign(ofdm_num: int):

@@ -435,6 +435,50 @@ int mega_precode_syms(const mega_precode_desc* d, mega_stream_t stream) {
   return MEGA_OK;
 }
 
+int mega_ldpc_decode(const mega_ldpc_desc* d, mega_stream_t stream) {
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, "mega_ldpc_decode: desc is NULL");
+  if (d->base_graph != 1 && d->base_graph != 2)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_ldpc_decode: base_graph must be 1 or 2");
+  if (mega::detail::ldpc_set_index(d->lifting_size) < 0)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_ldpc_decode: lifting size " +
+                                          std::to_string(d->lifting_size) +
+                                          " is not a 5G NR lifting size");
+  const uint32_t kb = d->base_graph == 1 ? 22u : 10u;
+  const uint32_t max_mb = d->base_graph == 1 ? 46u : 42u;
+  if (d->parity_nodes < 4 || d->parity_nodes > max_mb)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_ldpc_decode: parity_nodes out of range");
+  if (d->max_iterations < 1 || d->max_iterations > 100)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_ldpc_decode: max_iterations must be 1..100");
+  if (d->llr_bits != 16 && d->llr_bits != 32)
+    return fail(MEGA_ERR_UNSUPPORTED, "mega_ldpc_decode: llr_bits must be 16 or 32");
+  if (!(d->normalization >= 0.f && d->normalization <= 1.f))
+    return fail(MEGA_ERR_INVALID_ARG, "mega_ldpc_decode: normalization must be in [0, 1]");
+  if (d->num_cb == 0) return MEGA_OK;
+  if (d->num_cb > 0x7fffffffu) return fail(MEGA_ERR_UNSUPPORTED, "mega_ldpc_decode: num_cb too large");
+  if (!d->llr || !d->out) return fail(MEGA_ERR_INVALID_ARG, "mega_ldpc_decode: NULL buffer");
+  const uint64_t nv = (uint64_t)(kb + d->parity_nodes) * d->lifting_size;
+  if (d->llr_stride < nv)
+    return fail(MEGA_ERR_INVALID_ARG, "mega_ldpc_decode: llr_stride < (Kb + mb) * Zc");
+  if (d->out_stride < ((uint64_t)kb * d->lifting_size + 31) / 32)
+    return fail(MEGA_ERR_INVALID_ARG, "mega_ldpc_decode: out_stride < ceil(Kb * Zc / 32)");
+  mega::detail::LdpcArgs a;
+  a.bg = d->base_graph;
+  a.zc = d->lifting_size;
+  a.mb = d->parity_nodes;
+  a.iters = d->max_iterations;
+  a.num_cb = d->num_cb;
+  a.llr_bits = d->llr_bits;
+  a.alpha = d->normalization > 0.f ? d->normalization : 0.8125f;
+  a.llr = d->llr;
+  a.llr_stride = d->llr_stride;
+  a.out = d->out;
+  a.out_stride = d->out_stride;
+  const char* why = nullptr;
+  hipError_t e = mega::detail::launch_ldpc_decode(a, static_cast<hipStream_t>(stream), &why);
+  if (e != hipSuccess) return hip_fail(e, "mega_ldpc_decode", why);
+  return MEGA_OK;
+}
+
 int mega_precode(uint64_t row, uint64_t col, uint64_t batch_count, const void* precode,
                  const void* uevec, void* bsvec, uint64_t bs_stride, uint64_t precode_skip,
                  uint64_t bs_start, mega_stream_t stream) {

@@ -101,6 +101,22 @@ struct FftArgs {
 // mode: 0 uplink, 1 pilot, 2 downlink IFFT
 hipError_t launch_fft(const FftArgs& a, int mode, hipStream_t stream, const char** why);
 
+// LDPC decoding (ldpc.hip): num_cb code blocks of base graph bg (1 or 2),
+// lifting size zc, mb parity block rows, iters layered min-sum iterations
+// with scale alpha; llr binary16 (llr_bits 16) or float (32), llr_stride
+// elements between code blocks; out bit words, out_stride words apart.
+struct LdpcArgs {
+  uint32_t bg, zc, mb, iters, num_cb;
+  int32_t llr_bits;
+  float alpha;
+  const void* llr;
+  uint64_t llr_stride;
+  uint32_t* out;
+  uint64_t out_stride;
+};
+int ldpc_set_index(uint32_t zc);
+hipError_t launch_ldpc_decode(const LdpcArgs& a, hipStream_t stream, const char** why);
+
 hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
                                const float2* in, float2* out,
                                hipStream_t stream);

@@ -29,7 +29,8 @@ def build() -> str:
     cmd = [
         "gcc", "-O3", "-std=c11", "-fPIC", "-shared", "-fopenmp",
         "-ffp-contract=off", "-fno-fast-math", "-Wall",
-        "-o", LIB_PATH + ".tmp", os.path.join(_HERE, "oracle.c"), "-lm",
+        "-o", LIB_PATH + ".tmp", os.path.join(_HERE, "oracle.c"),
+        os.path.join(_HERE, "ldpc.c"), "-lm",
     ]
     subprocess.run(cmd, check=True)
     os.replace(LIB_PATH + ".tmp", LIB_PATH)

@@ -8,7 +8,8 @@
 //
 // Mirrors test/matrix/test_matrix.cc, test/matrix/test_cuphy_tensor.cc,
 // test/modulation/test_modulation.cc:11-57, test/sgemv/test_sgemv.cc:126-196,
-// test/fft/test_fft.cc:56-160 and test/beamform/test_beamform.cc:67-95.
+// test/fft/test_fft.cc:56-160, test/beamform/test_beamform.cc:67-95 and
+// test/ldpc/test_ldpc.cc:75-88,155-214 (configuration and decoder tests).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -23,6 +24,7 @@
 #include "mega/demodulation.h"
 #include "mega/equalize.h"
 #include "mega/fft.h"
+#include "mega/ldpc.h"
 #include "mega/matrix.h"
 #include "mega/mega_complex.h"
 #include "mega/padded_tensor.h"
@@ -313,6 +315,52 @@ static void beamform_fixture(const std::string& dir) {
   EXPECT_THROW(mega::Beamform::beamform_downlink(csi, bad), std::runtime_error);
 }
 
+// test_ldpc.cc:75-88: the configuration's derived sizes.
+static void ldpc_config_test() {
+  std::printf("LDPCConfig\n");
+  mega::LDPCConfig config(46, 1, 64, 8);
+  EXPECT(config.parity_nodes == 46 && config.base_graph == 1);
+  EXPECT(config.lifting_factor == 64 && config.max_decode_iter == 8);
+  EXPECT(config.info_nodes == 22);
+  EXPECT(config.encoded_bits == 4224 && config.decoded_bits == 1408);
+  EXPECT(config.punctured_bits == 128);
+  mega::LDPCConfig c2(42, 2, 384, 1);
+  EXPECT(c2.info_nodes == 10 && c2.encoded_bits == 50ull * 384 && c2.decoded_bits == 3840);
+}
+
+// test_ldpc.cc:155-214: noiseless float LLRs (+-127 after the punctured
+// zero prefix) of encode.data decode to input.data, one code block and two.
+static void ldpc_fixture(const std::string& dir) {
+  std::printf("LDPCDecoder\n");
+  const auto enc = read_file<unsigned char>(dir + "/ldpc_encoded_bits.bin");
+  const auto inp = read_file<unsigned char>(dir + "/ldpc_input_bits.bin");
+  mega::LDPCConfig config(46, 1, 64, 8);
+  EXPECT(enc.size() == 2 * config.encoded_bits / 8 && inp.size() == 2 * config.decoded_bits / 8);
+  mega::LDPCDecoder decoder(CuphyTensor::kFloat, config);
+  const uint32_t nllr = (uint32_t)(config.encoded_bits + config.punctured_bits);
+  for (uint32_t ncb = 1; ncb <= 2; ++ncb) {
+    std::vector<float> llr((size_t)nllr * ncb, 0.f);
+    for (uint32_t c = 0; c < ncb; ++c)
+      for (uint64_t i = 0; i < config.encoded_bits; ++i) {
+        const uint64_t bit = c * config.encoded_bits + i;
+        const bool one = enc[bit / 8] & (1u << (bit % 8));
+        llr[(size_t)c * nllr + config.punctured_bits + i] = one ? -127.f : 127.f;
+      }
+    CuphyTensor d_llr(CuphyTensor::kFloat, nllr, ncb, Matrix::kDevice);
+    CuphyTensor d_out(CuphyTensor::kBit, (uint32_t)config.decoded_bits, ncb, Matrix::kDevice);
+    EXPECT(d_out.szBytes() == ncb * config.decoded_bits / 8);
+    EXPECT(hipMemcpy(d_llr.ptr(), llr.data(), d_llr.szBytes(), hipMemcpyHostToDevice) == hipSuccess);
+    decoder(d_llr, d_out);
+    std::vector<unsigned char> out(d_out.szBytes());
+    EXPECT(hipMemcpy(out.data(), d_out.ptr(), out.size(), hipMemcpyDeviceToHost) == hipSuccess);
+    EXPECT(std::memcmp(out.data(), inp.data(), out.size()) == 0);
+  }
+  // the LLR type must match the decoder's
+  CuphyTensor half_llr(CuphyTensor::kHalf, nllr, Matrix::kDesc);
+  CuphyTensor out1(CuphyTensor::kBit, (uint32_t)config.decoded_bits, Matrix::kDesc);
+  EXPECT_THROW(decoder(half_llr, out1), std::invalid_argument);
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "host";
   try {
@@ -327,7 +375,9 @@ int main(int argc, char** argv) {
     matrix_tests();
     tensor_tests();
     demod_known_answer();
+    ldpc_config_test();
     if (mode == "gpu") {
+      ldpc_fixture(argv[2]);
       equalize_fixture(argv[2]);
       precode_fixture(argv[2]);
       fft_fixtures(argv[2]);

@@ -30,6 +30,12 @@ Sources (binary fixtures next to the reference tests):
   test/fft/zachu_shift.data Zadoff-Chu of 16 times the pi/4 cyclic shift (:32-54)
   test/fft/ifft_in.data     downlink IFFT input, [bs=8][32] complex64 (:162-206)
   test/fft/ifft_out.data    expected int16 (re, im) pairs, [bs=8][32] (exact)
+  test/ldpc/input.data      two code blocks of information bits, BG1, Zc = 64,
+                            22 * 64 = 1408 bits each, bit-packed LSB first
+                            (test/ldpc/test_ldpc.cc:41-60)
+  test/ldpc/encode.data     their codewords without the 2 * Zc punctured bits,
+                            (22 - 2 + 46) * 64 = 4224 bits each, same packing
+                            (:62-73, :155-190)
 
 The arrays are stored unchanged (raw little-endian values); this script only
 re-packages them so the tests run where the reference tree is absent.
@@ -51,6 +57,8 @@ def rd(rel, dtype):
 
 def main():
     arrays = {
+        "ldpc_input_bits": rd("test/ldpc/input.data", np.uint8),
+        "ldpc_encoded_bits": rd("test/ldpc/encode.data", np.uint8),
         "sgemv_csi": rd("test/sgemv/csi.data", np.complex64),
         "sgemv_bsvec_kb": rd("test/sgemv/bsVec.data", np.complex64),
         "sgemv_equalize_f16bits": rd("test/sgemv/equalize.data", np.uint16),

@@ -148,3 +148,30 @@ def test_host_demod_matches_oracle(orc, Q):
     assert np.array_equal(m.demod_host(x, Q), orc.demod(x, Q))
     with pytest.raises(m.MegaError):
         m.demod_host(x[:4], 5)
+
+
+def test_bad_ldpc_arguments_rejected_before_launch():
+    """mega_ldpc_decode checks the base graph, lifting size, parity rows,
+    iterations, LLR type, scale and strides before any device work."""
+    L = m.lib()
+    base = dict(base_graph=1, lifting_size=64, parity_nodes=46, max_iterations=8,
+                normalization=0.0, llr_bits=32, llr=1, llr_stride=68 * 64, num_cb=1, out=1,
+                out_stride=44)
+
+    def call(**kw):
+        d = m.LdpcDesc(**dict(base, **kw))
+        return L.mega_ldpc_decode(ctypes.byref(d), None)
+    for kw, code, msg in ((dict(base_graph=0), m.MEGA_ERR_UNSUPPORTED, "base_graph"),
+                          (dict(lifting_size=17), m.MEGA_ERR_UNSUPPORTED, "lifting size 17"),
+                          (dict(parity_nodes=3), m.MEGA_ERR_UNSUPPORTED, "parity_nodes"),
+                          (dict(base_graph=2, parity_nodes=43), m.MEGA_ERR_UNSUPPORTED,
+                           "parity_nodes"),
+                          (dict(max_iterations=101), m.MEGA_ERR_UNSUPPORTED, "max_iterations"),
+                          (dict(llr_bits=8), m.MEGA_ERR_UNSUPPORTED, "llr_bits"),
+                          (dict(normalization=1.5), m.MEGA_ERR_INVALID_ARG, "normalization"),
+                          (dict(llr=None), m.MEGA_ERR_INVALID_ARG, "NULL"),
+                          (dict(llr_stride=68 * 64 - 1), m.MEGA_ERR_INVALID_ARG, "llr_stride"),
+                          (dict(out_stride=43), m.MEGA_ERR_INVALID_ARG, "out_stride")):
+        assert call(**kw) == code, kw
+        assert msg in m.last_error(), (kw, m.last_error())
+    assert call(num_cb=0) == m.MEGA_OK          # nothing to decode
