@@ -39,11 +39,13 @@ extern "C" {
 
 /* Numerical form of the ZF / equalize kernels. */
 #define MEGA_FORM_FP32 0   /* plain FP32 arithmetic: the drop-in form */
-/* Products on the BF16 matrix cores with each FP32 operand split into a
- * hi + lo BF16 pair (hi*hi + hi*lo + lo*hi, FP32 accumulation); FP32 sweep
- * inverse with one refinement step whose residual is formed in FP64; groups
- * whose Gram matrix has 1-norm condition number above 1e3 are computed in
- * FP64 throughout. mega_zf and mega_detect_slots only. */
+/* Products on the BF16 matrix cores with FP32 operands split into BF16
+ * pieces, FP32 accumulation: the Gram matrix and W = X A on three-piece
+ * splits (FP32-accurate products, because the condition number multiplies
+ * their errors), the equalize on hi + lo pairs (hi*hi + hi*lo + lo*hi);
+ * FP32 sweep inverse with one refinement step whose residual is formed in
+ * FP64; groups whose Gram matrix has 1-norm condition number above 1e3 are
+ * computed in FP64 throughout. mega_zf and mega_detect_slots only. */
 #define MEGA_FORM_MIXED 1
 
 typedef void *mega_stream_t; /* hipStream_t */

@@ -76,5 +76,33 @@ __device__ __forceinline__ v4f mfma_x3(const v4u& ah, const v4u& al, const v4u& 
   return mfma_bf16(ah, bh, acc);
 }
 
+// Three-piece split, for products whose errors a condition number amplifies
+// (the Gram matrix that the inverse is refined against, W = X A): x = hi + lo
+// + rest with rest = RN_bf16((x - hi) - lo), both differences exact in FP32.
+// The pieces carry about 24 significant bits; a product taken as
+//   hi*hi + (lo*hi + hi*lo) + (lo*lo + rest*hi + hi*rest)
+// drops terms below 2^-25 of it, under the FP32 accumulation's own rounding.
+struct SplitC3 {
+  uint32_t hi, lo, rest;
+};
+
+__device__ __forceinline__ SplitC3 split_c3(float2 z) {
+  const SplitC s = split_c(z);
+  const float hre = __uint_as_float(s.hi << 16), him = __uint_as_float(s.hi & 0xffff0000u);
+  const float lre = __uint_as_float(s.lo << 16), lim = __uint_as_float(s.lo & 0xffff0000u);
+  const v2bf r = __builtin_convertvector((v2f){(z.x - hre) - lre, (z.y - him) - lim}, v2bf);
+  return SplitC3{s.hi, s.lo, __builtin_bit_cast(uint32_t, r)};
+}
+
+// acc += X Z over one K step of 16 complex indices, both split in three;
+// the small terms first.
+__device__ __forceinline__ v4f mfma_x6(const v4u& ah, const v4u& al, const v4u& ar,
+                                       const v4u& bh, const v4u& bl, const v4u& br, v4f acc) {
+  acc = mfma_bf16(al, bl, acc);
+  acc = mfma_bf16(ar, bh, acc);
+  acc = mfma_bf16(ah, br, acc);
+  return mfma_x3(ah, al, bh, bl, acc);
+}
+
 }  // namespace detail
 }  // namespace mega

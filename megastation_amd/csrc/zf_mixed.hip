@@ -1,13 +1,19 @@
 // Mixed-precision zero-forcing weights (the precision form of the detection
 // path): per subcarrier group, one workgroup (one wave for up to 16 users,
 // else four) computes
-//   1. G = A A^H on the BF16 matrix cores with split operands (prec_core.h),
-//      FP32 accumulation, mirrored to an exactly Hermitian matrix;
+//   1. G = A A^H on the BF16 matrix cores with operands split in three
+//      (prec_core.h; FP32-accurate products), FP32 accumulation, mirrored
+//      to an exactly Hermitian matrix;
 //   2. X0 = G^-1 by the FP32 sweep operator (zf_core.h), and the 1-norm
 //      condition number kappa = |G|_1 |X0|_1 (exact: X0 is explicit);
 //   3. one step of iterative refinement of the inverse,
 //        R = I - G X0 (products and sums in FP64),  X = X0 + X0 R (FP32);
-//   4. W = X A on the BF16 matrix cores with split operands.
+//   4. W = X A on the BF16 matrix cores, operands split in three.
+// Both products carry FP32 accuracy because the condition number multiplies
+// their errors into x_hat: with two-piece operands (about 16 bits) the Gram
+// matrix the inverse is refined against, and W, would err by kappa * 2^-17,
+// about 4e-7 * kappa_1 relative to max |x_hat| at 32 x 64; with three pieces
+// the error is that of FP32 arithmetic.
 // Groups with kappa above kCondMax, or whose FP32 sweep meets a pivot at or
 // below the relative tolerance, take the FP64 route instead: Gram, sweep and
 // W = G^-1 A all in FP64 (vector ALU), rounded to FP32 at the end; a pivot
@@ -159,15 +165,18 @@ __device__ __forceinline__ void store_pad(uint32_t* ws, uint32_t U, uint32_t Bp,
 // ---- Gram on split BF16 products ------------------------------------------
 // Antenna chunks (stage_kc) go through LDS once per group as split words:
 // every thread loads its share of kStageBatch chunks at once (coalesced, the
-// whole batch in flight), then per chunk splits it once and writes the hi
-// and lo planes [user][antenna] (rows of 4 more words, 16-byte aligned),
+// whole batch in flight), then per chunk splits it once in three and writes
+// the hi, lo and rest planes [user][antenna] (rows of 4 more words, 16-byte
+// aligned),
 // double-buffered with one barrier per chunk. Output tiles are 8 users (rows, re/im
 // interleaved) x 16 users (columns); only tiles reaching the lower triangle
 // are computed (the caller mirrors), spread over the four waves.
-// Chunk of antennas staged at a time: 32 for up to 16 users (small LDS
-// footprint, many groups per CU), else 64; rows padded by 4 words.
+// Chunk of antennas staged at a time: 64 for 17 to 32 users, else 32 (a
+// small footprint for up to 16 users, many groups per CU; LDS room for 64
+// users); rows padded by 4 words.
 template <int UT>
-__host__ __device__ constexpr int stage_kc() { return UT == 1 ? 32 : 64; }
+__host__ __device__ constexpr int stage_kc() { return UT == 2 ? 64 : 32; }
+constexpr int kPieces = 3;  // hi, lo, rest
 constexpr int kStageBatch = 4;
 
 // Staging buffers: one for one-wave groups (the smallest LDS footprint), two
@@ -177,7 +186,7 @@ __host__ __device__ constexpr int stage_bufs() { return UT == 1 ? 1 : 2; }
 
 template <int UT>
 __host__ __device__ constexpr size_t gram_stage_bytes() {
-  return (size_t)stage_bufs<UT>() * 2 * 16 * UT * (stage_kc<UT>() + 4) * sizeof(uint32_t);
+  return (size_t)stage_bufs<UT>() * kPieces * 16 * UT * (stage_kc<UT>() + 4) * sizeof(uint32_t);
 }
 
 template <int UT>
@@ -213,7 +222,7 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
   const int i16 = lane & 15, q4 = lane >> 4;
   const ARow h(i16 & 1);
   const uint32_t nch = (B + kStageKC - 1) / kStageKC;
-  auto plane = [&](int buf, int lo) { return stage + (2 * buf + lo) * UP * kStageKCP; };
+  auto plane = [&](int buf, int piece) { return stage + (kPieces * buf + piece) * UP * kStageKCP; };
 
   // all chunks of a batch of up to kStageBatch are loaded at once, so the
   // group's A is in flight together (one HBM round trip per batch)
@@ -230,14 +239,16 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
   auto put = [&](uint32_t ch, const float2 (&d)[PER], int buf) {
     uint32_t* ph = plane(buf, 0);
     uint32_t* pl = plane(buf, 1);
+    uint32_t* pr = plane(buf, 2);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const uint32_t e = tid + NTH * j;
       const uint32_t u = e % UP, bl = e / UP;
       const bool ok = u < U && ch * kStageKC + bl < B;
-      const SplitC sp = split_c(ok ? d[j] : make_float2(0.f, 0.f));
+      const SplitC3 sp = split_c3(ok ? d[j] : make_float2(0.f, 0.f));
       ph[u * kStageKCP + bl] = sp.hi;
       pl[u * kStageKCP + bl] = sp.lo;
+      pr[u * kStageKCP + bl] = sp.rest;
     }
   };
 
@@ -253,28 +264,34 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
     tcol[t] = rc < 0 ? -1 : 16 * (rc % T::CT);
   }
   // Per 16-antenna step, the operands of all the wave's tiles are read
-  // first, then the three products of every tile go to separate
-  // accumulators (no dependent chains; summed at the end).
+  // first, then the six products of every tile go to three accumulators by
+  // size (hi*hi; lo*hi + hi*lo; lo*lo + rest*hi + hi*rest), summed at the
+  // end, smallest first.
   auto products = [&](int buf) {
     const uint32_t* ph = plane(buf, 0);
     const uint32_t* pl = plane(buf, 1);
+    const uint32_t* pr = plane(buf, 2);
 #pragma unroll
     for (int st = 0; st < kStageKC / 16; ++st) {
       const int k0 = 16 * st + 4 * q4;
-      v4u ah[TW], al[TW], bh[TW], bl[TW];
+      v4u ah[TW], al[TW], ar[TW], bh[TW], bl[TW], br[TW];
 #pragma unroll
       for (int t = 0; t < TW; ++t) {
         const uint32_t u = max(trow[t], 0) + (i16 >> 1), v = max(tcol[t], 0) + i16;
         const v4u uh = *reinterpret_cast<const v4u*>(ph + u * kStageKCP + k0);
         const v4u ul = *reinterpret_cast<const v4u*>(pl + u * kStageKCP + k0);
+        const v4u ur = *reinterpret_cast<const v4u*>(pr + u * kStageKCP + k0);
         const v4u vh = *reinterpret_cast<const v4u*>(ph + v * kStageKCP + k0);
         const v4u vl = *reinterpret_cast<const v4u*>(pl + v * kStageKCP + k0);
+        const v4u vr = *reinterpret_cast<const v4u*>(pr + v * kStageKCP + k0);
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           ah[t][a] = a_word(uh[a], h);
           al[t][a] = a_word(ul[a], h);
+          ar[t][a] = a_word(ur[a], h);
           bh[t][a] = vh[a] ^ 0x80000000u;  // conj(A[v][b])
           bl[t][a] = vl[a] ^ 0x80000000u;
+          br[t][a] = vr[a] ^ 0x80000000u;
         }
       }
 #pragma unroll
@@ -282,7 +299,10 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
         if (trow[t] < 0) continue;  // uniform per wave
         acc[t] = mfma_bf16(ah[t], bh[t], acc[t]);
         acc2[t] = mfma_bf16(al[t], bh[t], acc2[t]);
-        acc3[t] = mfma_bf16(ah[t], bl[t], acc3[t]);
+        acc3[t] = mfma_bf16(al[t], bl[t], acc3[t]);
+        acc2[t] = mfma_bf16(ah[t], bl[t], acc2[t]);
+        acc3[t] = mfma_bf16(ar[t], bh[t], acc3[t]);
+        acc3[t] = mfma_bf16(ah[t], br[t], acc3[t]);
       }
     }
   };
@@ -306,7 +326,7 @@ __device__ __forceinline__ void gram_split(const float2* __restrict__ A, uint32_
   static_for<0, TW>([&](auto tc) {
     constexpr int t = decltype(tc)::value;
     if (trow[t] < 0) return;
-    acc[t] += acc2[t] + acc3[t];  // hi*hi + (lo*hi + hi*lo)
+    acc[t] += acc2[t] + acc3[t];  // hi*hi + ((lo*hi + hi*lo) + small terms)
     const uint32_t col = tcol[t] + i16;
 #pragma unroll
     for (int r2 = 0; r2 < 2; ++r2) {
@@ -442,8 +462,10 @@ __global__ __launch_bounds__(mix_threads<UT>(), UT == 1 ? 16 : UT == 2 ? 2 : 1) 
         }
       }
       __syncthreads();  // every read of G0 (R) is done
-      // split X into BF16 words for the apply (G0 reused as uint2 storage)
+      // split X in three for the apply: (hi, lo) words in G0 (reused as uint2
+      // storage), the rest words in the staging area (free since the Gram)
       uint2* sXw = reinterpret_cast<uint2*>(sh.G0);
+      uint32_t* sXr = reinterpret_cast<uint32_t*>(sM64);
 #pragma unroll
       for (int q = 0; q < (TT + NW - 1) / NW; ++q) {
         const int tile = wave + NW * q;
@@ -451,15 +473,17 @@ __global__ __launch_bounds__(mix_threads<UT>(), UT == 1 ? 16 : UT == 2 ? 2 : 1) 
         const int ti = tile / UT, tj = tile % UT;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const SplitC sp = split_c(xn[q][r]);
+          const SplitC3 sp = split_c3(xn[q][r]);
           sXw[(16 * ti + 4 * q4 + r) * LDG + 16 * tj + i16] = make_uint2(sp.hi, sp.lo);
+          sXr[(16 * ti + 4 * q4 + r) * LDG + 16 * tj + i16] = sp.rest;
         }
       }
       __syncthreads();
     }
     const uint2* sX = reinterpret_cast<const uint2*>(sh.G0);
+    const uint32_t* sXr = reinterpret_cast<const uint32_t*>(sM64);
 
-    // ---- 4. W = X A on split BF16 products ----------------------------------
+    // ---- 4. W = X A on products of three-piece splits -------------------------
     // A tiles of 16 antennas x all users; a wave's tiles are loaded in
     // batches of NAB, all in flight together, then multiplied and stored.
     const uint32_t ntiles = (B + 15) / 16;
@@ -492,15 +516,16 @@ __global__ __launch_bounds__(mix_threads<UT>(), UT == 1 ? 16 : UT == 2 ? 2 : 1) 
         const uint32_t nt = n0 + r * NW;
         if (nt >= ntiles) break;  // uniform per wave
         const uint32_t b = 16 * nt + i16;
-        v4u bh[UT], bl[UT];
+        v4u bh[UT], bl[UT], br[UT];
 #pragma unroll
         for (int ks = 0; ks < UT; ++ks)
 #pragma unroll
           for (int a = 0; a < 4; ++a) {
             const bool ok = 16u * ks + 4 * q4 + a < U;
-            const SplitC sp = split_c(ok ? zb[r][ks][a] : make_float2(0.f, 0.f));
+            const SplitC3 sp = split_c3(ok ? zb[r][ks][a] : make_float2(0.f, 0.f));
             bh[ks][a] = sp.hi;
             bl[ks][a] = sp.lo;
+            br[ks][a] = sp.rest;
           }
 #pragma unroll
         for (int mt = 0; mt < 2 * UT; ++mt) {
@@ -508,14 +533,15 @@ __global__ __launch_bounds__(mix_threads<UT>(), UT == 1 ? 16 : UT == 2 ? 2 : 1) 
           v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int ks = 0; ks < UT; ++ks) {
-            v4u ah, al;
+            v4u ah, al, ar;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
               const uint2 xw = sX[u * LDG + 16 * ks + 4 * q4 + a];
               ah[a] = a_word(xw.x, h);
               al[a] = a_word(xw.y, h);
+              ar[a] = a_word(sXr[u * LDG + 16 * ks + 4 * q4 + a], h);
             }
-            acc = mfma_x3(ah, al, bh[ks], bl[ks], acc);
+            acc = mfma_x6(ah, al, ar, bh[ks], bl[ks], br[ks], acc);
           }
           const uint32_t u0 = 8 * mt + 2 * q4;
           if (b < B) {
