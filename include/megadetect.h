@@ -285,6 +285,19 @@ int mega_detect_slots(const mega_detect_desc *desc, mega_stream_t stream);
  * rounded up to a multiple of 16). */
 uint64_t mega_detect_work_bytes(const mega_detect_desc *desc);
 
+/* mega_detect_slots that allocates nothing and takes no work buffer: the
+ * mixed form's split weight planes go to a static device array of the
+ * library, MEGA_DETECT_SCRATCH_BYTES per GPU, held until the process ends
+ * (its address is looked up at the first call on each device). desc->work
+ * must be NULL; a desc whose mega_detect_work_bytes exceeds the maximum is
+ * MEGA_ERR_UNSUPPORTED and launches nothing. The FP32 form needs no scratch
+ * and runs as mega_detect_slots. Calls that overlap on one GPU share the
+ * array and race: keep them on one stream. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). */
+#define MEGA_DETECT_SCRATCH_BYTES (32ull << 20)
+int mega_detect_slots_noalloc(const mega_detect_desc *desc, mega_stream_t stream);
+
 /* ---- Batching form: many uneven cells, one graph launch ----------------------
  * One cell of a batch: its own antenna x user size, subcarriers, group size,
  * symbols, modulation order and LLR layout (as mega_equalize_slots for one

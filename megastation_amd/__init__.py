@@ -151,6 +151,7 @@ def lib() -> ctypes.CDLL:
                            u64, P], i32),
         "mega_equalize_slots": ([ctypes.POINTER(EqualizeDesc), P], i32),
         "mega_detect_slots": ([ctypes.POINTER(DetectDesc), P], i32),
+        "mega_detect_slots_noalloc": ([ctypes.POINTER(DetectDesc), P], i32),
         "mega_detect_work_bytes": ([ctypes.POINTER(DetectDesc)], u64),
         "mega_demod_host": ([P, u64, u8, P], i32),
         "mega_precode": ([u64, u64, u64, P, P, P, u64, u64, u64, P], i32),
@@ -406,11 +407,13 @@ def detect_slots(csi, y, llr, *, num_ue, num_bs, num_sc, sc_group, num_syms,
                  valid_bits, csi_slot_stride, y_sym_stride, y_slot_stride,
                  llr_sym_stride, llr_slot_stride, xhat=None, w_out=None,
                  info=None, form: int = FORM_FP32, work=None, route=None,
-                 stream=None):
+                 stream=None, noalloc: bool = False):
     """Fused ZF + equalize + demod: csi holds the channel estimates A = H^H.
     form=FORM_MIXED needs `work` (detect_work_bytes(...) bytes of device
     memory, e.g. a torch uint8 tensor); `route` (int32, num_slots*G) gets 1
-    for groups computed on the FP64 route."""
+    for groups computed on the FP64 route. noalloc=True calls
+    mega_detect_slots_noalloc instead: no `work` (the library's static
+    scratch of DETECT_SCRATCH_BYTES per GPU holds the split weights)."""
     e = EqualizeDesc(num_ue, num_bs, num_sc, sc_group, num_syms, num_slots,
                      mod_order, form, ue_stride, cb_size, cb_stride, zeros,
                      valid_bits, _ptr(csi).value, csi_slot_stride, _ptr(y).value,
@@ -418,7 +421,11 @@ def detect_slots(csi, y, llr, *, num_ue, num_bs, num_sc, sc_group, num_syms,
                      llr_slot_stride, _ptr(xhat).value)
     d = DetectDesc(e, _ptr(w_out).value, _ptr(info).value, _ptr(work).value,
                    _ptr(route).value)
-    _check(lib().mega_detect_slots(ctypes.byref(d), _stream(stream)))
+    fn = lib().mega_detect_slots_noalloc if noalloc else lib().mega_detect_slots
+    _check(fn(ctypes.byref(d), _stream(stream)))
+
+
+DETECT_SCRATCH_BYTES = 32 << 20  # MEGA_DETECT_SCRATCH_BYTES
 
 
 def detect_work_bytes(*, num_ue, num_bs, num_sc, sc_group, num_slots,

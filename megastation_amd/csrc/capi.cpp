@@ -246,30 +246,57 @@ int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
   return MEGA_OK;
 }
 
-int mega_detect_slots(const mega_detect_desc* d, mega_stream_t stream) {
-  if (!d) return fail(MEGA_ERR_INVALID_ARG, "mega_detect_slots: desc is NULL");
+static int detect_checked(const mega_detect_desc* d, void* work, mega_stream_t stream,
+                          const char* who) {
   mega::detail::EqualizeArgs a;
   bool empty;
-  int r = eq_args_checked(&d->eq, a, empty, "mega_detect_slots", true);
+  int r = eq_args_checked(&d->eq, a, empty, who, true);
   if (r != MEGA_OK || empty) return r;
   if (d->eq.form == MEGA_FORM_MIXED) {
-    if (!d->work) return fail(MEGA_ERR_INVALID_ARG, "mega_detect_slots: the mixed form needs work");
+    if (!work) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": the mixed form needs work");
     const char* why = nullptr;
     hipError_t e = mega::detail::launch_detect_mixed(
-        a, static_cast<float2*>(d->w_out), d->work, d->info_dev, d->route_dev,
+        a, static_cast<float2*>(d->w_out), work, d->info_dev, d->route_dev,
         static_cast<hipStream_t>(stream), &why);
-    if (e != hipSuccess) return hip_fail(e, "mega_detect_slots", why);
+    if (e != hipSuccess) return hip_fail(e, who, why);
     return MEGA_OK;
   }
   if (!d->w_out && !mega::detail::detect_fused_supported(a.U, a.B))
-    return fail(MEGA_ERR_INVALID_ARG,
-                "mega_detect_slots: U > 16 or B > 256 needs w_out");
+    return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": U > 16 or B > 256 needs w_out");
   const char* why = nullptr;
   hipError_t e = mega::detail::launch_detect(a, static_cast<float2*>(d->w_out),
                                              d->info_dev,
                                              static_cast<hipStream_t>(stream), &why);
-  if (e != hipSuccess) return hip_fail(e, "mega_detect_slots", why);
+  if (e != hipSuccess) return hip_fail(e, who, why);
   return MEGA_OK;
+}
+
+int mega_detect_slots(const mega_detect_desc* d, mega_stream_t stream) {
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, "mega_detect_slots: desc is NULL");
+  return detect_checked(d, d->work, stream, "mega_detect_slots");
+}
+
+int mega_detect_slots_noalloc(const mega_detect_desc* d, mega_stream_t stream) {
+  static const char* who = "mega_detect_slots_noalloc";
+  if (!d) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": desc is NULL");
+  if (d->work)
+    return fail(MEGA_ERR_INVALID_ARG,
+                std::string(who) + ": work must be NULL (the library's scratch is used)");
+  if (d->eq.form != MEGA_FORM_MIXED) return detect_checked(d, nullptr, stream, who);
+  const uint64_t need = mega_detect_work_bytes(d);
+  if (need > mega::detail::kDetectScratchBytes)
+    return fail(MEGA_ERR_UNSUPPORTED,
+                std::string(who) + ": needs " + std::to_string(need) +
+                    " bytes of scratch, more than its maximum of " +
+                    std::to_string(mega::detail::kDetectScratchBytes) +
+                    " (fewer slots per call, or mega_detect_slots with a work buffer)");
+  void* work = nullptr;
+  if (need > 0) {
+    const char* why = nullptr;
+    hipError_t e = mega::detail::detect_scratch(&work, &why);
+    if (e != hipSuccess) return hip_fail(e, who, why);
+  }
+  return detect_checked(d, work, stream, who);
 }
 
 uint64_t mega_detect_work_bytes(const mega_detect_desc* d) {

@@ -15,6 +15,7 @@
 // of 64 antennas); other shapes run the two kernels back to back through
 // w_out.
 #include <cstdlib>
+#include <mutex>
 
 #include "zf_core.h"
 
@@ -185,6 +186,40 @@ hipError_t launch_detect_mixed(const EqualizeArgs& a, float2* w_out, void* work,
   EqualizeArgs b = a;
   b.csi_slot = per_slot;
   return launch_equalize_mixed(b, work, st, why);
+}
+
+// Scratch of the no-allocation entry point: the mixed form's split weight
+// planes. Every GPU the library runs on holds its own copy for the life of
+// the process. The array's name is not a device address in host code, so
+// the address is looked up once per device, with that device current, and
+// only that address is handed to kernels. Calls that overlap on one GPU
+// share the array and race; the kernels write every word they later read
+// (pads included), so nothing needs clearing between calls.
+static __device__ __align__(256) uint8_t scratch[kDetectScratchBytes];
+
+hipError_t detect_scratch(void** out, const char** why) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) {
+    *why = "device index above 63";
+    return hipErrorInvalidDevice;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *why = "hipGetSymbolAddress of the scratch array";
+      return e != hipSuccess ? e : hipErrorInvalidSymbol;
+    }
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return hipSuccess;
 }
 
 }  // namespace detail

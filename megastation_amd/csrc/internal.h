@@ -53,6 +53,10 @@ hipError_t launch_equalize_mixed(const EqualizeArgs& a, const void* wp,
 hipError_t launch_detect_mixed(const EqualizeArgs& a, float2* w_out, void* work,
                                int32_t* info, int32_t* route, hipStream_t stream,
                                const char** why);
+// Device address, on the current device, of the library's static scratch for
+// mega_detect_slots_noalloc (kDetectScratchBytes; looked up once per device).
+constexpr uint64_t kDetectScratchBytes = 32ull << 20;
+hipError_t detect_scratch(void** out, const char** why);
 
 // Learned form (learned.hip): Gram and plain inverse per group; network
 // features; warm start + Newton-Schulz + residual check; final W with the
