@@ -1108,7 +1108,9 @@ def measured_traffic(workload, kernel_prefix):
     with open(path) as fh:
         data = json.load(fh).get(workload, {})
     for k, v in data.items():
-        if k.startswith(kernel_prefix):
+        # the FP32 equalize kernels (equalize_mfma / _stream / _wide), not the
+        # mixed form's
+        if k.startswith(kernel_prefix) and not k.startswith("equalize_mixed"):
             return {"traffic_bytes": v["traffic_bytes"],
                     "source": f"profiles/traffic.json: {k}, rocprofv3 FETCH_SIZE/WRITE_SIZE "
                               "scaled by a 1 GiB copy"}

@@ -39,6 +39,8 @@ struct mega_beamform {
   uint32_t num_bs, num_ue, batch_count;
   int device;
   int32_t* info;  // device, [batch_count]
+  void* work;     // device, the three-kernel ZF's workspace for batch_count (or NULL)
+  uint64_t work_bytes;
 };
 
 extern "C" {
@@ -55,12 +57,20 @@ int mega_beamform_create(uint32_t num_bs, uint32_t num_ue, uint32_t batch_count,
     return fail(MEGA_ERR_INVALID_ARG, "num_bs, num_ue and batch_count must be > 0");
   if (num_ue > mega::detail::kMaxUsers)
     return fail(MEGA_ERR_UNSUPPORTED, "num_ue > 64 is not supported");
-  auto* bf = new mega_beamform{num_bs, num_ue, batch_count, 0, nullptr};
+  auto* bf = new mega_beamform{num_bs, num_ue, batch_count, 0, nullptr, nullptr, 0};
   hipError_t e = hipGetDevice(&bf->device);
   if (e == hipSuccess) e = hipMalloc(&bf->info, sizeof(int32_t) * batch_count);
   if (e == hipSuccess) e = hipMemset(bf->info, 0, sizeof(int32_t) * batch_count);
+  // the workspace of the three-kernel ZF (few wide groups), like the
+  // reference's Gram workspace of batch_count groups (beamform.cu:19-31)
+  const uint64_t wb = mega::detail::zf_wide_work_bytes(num_ue, num_bs, batch_count);
+  if (e == hipSuccess && wb > 0 && mega::detail::zf_wide_pays(num_ue, num_bs, batch_count)) {
+    e = hipMalloc(&bf->work, wb);
+    if (e == hipSuccess) bf->work_bytes = wb;
+  }
   if (e != hipSuccess) {
     if (bf->info) (void)hipFree(bf->info);
+    if (bf->work) (void)hipFree(bf->work);
     delete bf;
     return hip_fail(e, "mega_beamform_create");
   }
@@ -71,6 +81,10 @@ int mega_beamform_create(uint32_t num_bs, uint32_t num_ue, uint32_t batch_count,
 int mega_beamform_destroy(mega_beamform_t bf) {
   if (!bf) return MEGA_OK;
   hipError_t e = hipFree(bf->info);
+  if (bf->work) {
+    const hipError_t e2 = hipFree(bf->work);
+    if (e == hipSuccess) e = e2;
+  }
   delete bf;
   if (e != hipSuccess) return hip_fail(e, "mega_beamform_destroy");
   return MEGA_OK;
@@ -78,7 +92,8 @@ int mega_beamform_destroy(mega_beamform_t bf) {
 
 static int zf_checked(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
                       const void* in, void* out, int32_t* info, int form,
-                      mega_stream_t stream, const char* who) {
+                      mega_stream_t stream, const char* who, void* work = nullptr,
+                      uint64_t work_bytes = 0) {
   if (form != MEGA_FORM_FP32 && form != MEGA_FORM_MIXED)
     return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": unsupported form " +
                                           std::to_string(form));
@@ -96,7 +111,7 @@ static int zf_checked(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
                                           static_cast<hipStream_t>(stream), &why)
           : mega::detail::launch_zf(num_ue, num_bs, batch, static_cast<const float2*>(in),
                                     static_cast<float2*>(out), info,
-                                    static_cast<hipStream_t>(stream), &why);
+                                    static_cast<hipStream_t>(stream), &why, work, work_bytes);
   if (e != hipSuccess) return hip_fail(e, who, why);
   return MEGA_OK;
 }
@@ -125,7 +140,7 @@ int mega_beamform_uplink(mega_beamform_t bf, void* csi, uint32_t batch,
                                     " exceeds the maximum batch size " +
                                     std::to_string(bf->batch_count));
   return zf_checked(bf->num_bs, bf->num_ue, batch, csi, csi, bf->info,
-                    MEGA_FORM_FP32, stream, "mega_beamform_uplink");
+                    MEGA_FORM_FP32, stream, "mega_beamform_uplink", bf->work, bf->work_bytes);
 }
 
 int mega_beamform_downlink(const void* csi, void* precode, uint32_t num_ue,
@@ -247,7 +262,7 @@ int mega_equalize_slots(const mega_equalize_desc* d, mega_stream_t stream) {
 }
 
 static int detect_checked(const mega_detect_desc* d, void* work, mega_stream_t stream,
-                          const char* who) {
+                          const char* who, bool may_alloc = true) {
   mega::detail::EqualizeArgs a;
   bool empty;
   int r = eq_args_checked(&d->eq, a, empty, who, true);
@@ -266,7 +281,8 @@ static int detect_checked(const mega_detect_desc* d, void* work, mega_stream_t s
   const char* why = nullptr;
   hipError_t e = mega::detail::launch_detect(a, static_cast<float2*>(d->w_out),
                                              d->info_dev,
-                                             static_cast<hipStream_t>(stream), &why);
+                                             static_cast<hipStream_t>(stream), &why,
+                                             may_alloc);
   if (e != hipSuccess) return hip_fail(e, who, why);
   return MEGA_OK;
 }
@@ -282,7 +298,9 @@ int mega_detect_slots_noalloc(const mega_detect_desc* d, mega_stream_t stream) {
   if (d->work)
     return fail(MEGA_ERR_INVALID_ARG,
                 std::string(who) + ": work must be NULL (the library's scratch is used)");
-  if (d->eq.form != MEGA_FORM_MIXED) return detect_checked(d, nullptr, stream, who);
+  // the FP32 form needs no scratch; its ZF takes the one-workgroup-per-group
+  // kernel here (the three-kernel form's workspace would be an allocation)
+  if (d->eq.form != MEGA_FORM_MIXED) return detect_checked(d, nullptr, stream, who, false);
   const uint64_t need = mega_detect_work_bytes(d);
   if (need > mega::detail::kDetectScratchBytes)
     return fail(MEGA_ERR_UNSUPPORTED,

@@ -104,7 +104,7 @@ hipError_t launch_q(const EqParams& p, dim3 grid, size_t lds, uint32_t Q,
 bool detect_fused_supported(uint32_t U, uint32_t B) { return U <= 16 && B <= 256; }
 
 hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
-                         hipStream_t st, const char** why) {
+                         hipStream_t st, const char** why, bool may_alloc) {
   if (a.U == 0 || a.B == 0 || a.N == 0 || a.sg == 0 || a.S == 0 || a.slots == 0)
     return hipSuccess;
   if (!detect_fused_supported(a.U, a.B)) {
@@ -114,11 +114,13 @@ hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
     }
     hipError_t e = hipSuccess;
     if (a.csi_slot == (uint64_t)a.G * a.U * a.B || a.slots == 1) {
-      e = launch_zf(a.U, a.B, a.slots * a.G, a.csi, w_out, info, st, why);
+      e = launch_zf(a.U, a.B, a.slots * a.G, a.csi, w_out, info, st, why, nullptr, 0,
+                    may_alloc);
     } else {  // padded slot stride (w_out has the same layout): one launch per slot
       for (uint32_t s = 0; s < a.slots && e == hipSuccess; ++s)
         e = launch_zf(a.U, a.B, a.G, a.csi + s * a.csi_slot, w_out + s * a.csi_slot,
-                      info ? info + (uint64_t)s * a.G : nullptr, st, why);
+                      info ? info + (uint64_t)s * a.G : nullptr, st, why, nullptr, 0,
+                      may_alloc);
     }
     if (e != hipSuccess) return e;
     EqualizeArgs b = a;

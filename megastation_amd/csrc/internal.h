@@ -27,15 +27,31 @@ struct EqualizeArgs {
 hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t stream,
                            const char** why);
 
+// FP32 ZF. Where zf_wide_pays (no shape at present: measured no faster) or
+// MEGA_ZF_WIDE=2 asks for it, 17 to 64 users take the three-kernel form
+// (zf_wide.hip) when a workspace of zf_wide_work_bytes is at hand:
+// `work` (work_bytes) if given, else the stream's library workspace when
+// may_alloc (never while the stream is being captured); otherwise, and for
+// every other shape, one workgroup per group (zf.hip). W and the status are
+// the same bits either way.
 hipError_t launch_zf(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
                      float2* out, int32_t* info, hipStream_t stream,
-                     const char** why);
+                     const char** why, void* work = nullptr, uint64_t work_bytes = 0,
+                     bool may_alloc = true);
+uint64_t zf_wide_work_bytes(uint32_t U, uint32_t B, uint32_t batch);
+bool zf_wide_pays(uint32_t U, uint32_t B, uint32_t batch);
+hipError_t launch_zf_wide(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
+                          float2* out, int32_t* info, void* work, hipStream_t stream,
+                          const char** why);
+// The library's device workspace of stream st (at least `need` bytes; see
+// workspace.cpp). hipErrorStreamCaptureUnsupported while st is capturing.
+hipError_t stream_workspace(hipStream_t st, size_t need, void** out);
 
 // ZF + equalize + demod of every symbol of every slot: fused in one kernel for
 // U <= 16 and B <= 256 (w_out optional), otherwise the ZF and equalize kernels
 // back to back through w_out (required then).
 hipError_t launch_detect(const EqualizeArgs& a, float2* w_out, int32_t* info,
-                         hipStream_t stream, const char** why);
+                         hipStream_t stream, const char** why, bool may_alloc = true);
 bool detect_fused_supported(uint32_t U, uint32_t B);
 
 // Mixed form (split-BF16 matrix-core products; zf_mixed.hip): W as FP32
@@ -130,7 +146,7 @@ hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
 // the launcher's own choice except for kDetPf (default 1).
 enum TuneKey {
   kEqKs, kEqMode, kEqSch, kEqSmt, kEqTpw, kEqXmt, kEqXreg, kEqXt, kEqXks, kPrecAcc, kDetPf,
-  kEqWtc, kZfNth,
+  kEqWtc, kZfNth, kZfWide,
   kTuneCount
 };
 int tune(TuneKey k);

@@ -33,6 +33,8 @@ struct mega_pipeline {
   hipStream_t st;  // caller's stream (equalize)
   hipStream_t zs;  // ZF stream
   std::vector<void*> w;
+  void* zf_work = nullptr;  // the three-kernel ZF's workspace (FP32 form, few wide groups)
+  uint64_t zf_work_bytes = 0;
   std::vector<hipEvent_t> zf_ev, eq_ev;
   std::vector<bool> eq_rec;
   uint64_t pushed = 0;
@@ -53,6 +55,7 @@ int hip_err(hipError_t e, const char* who, const char* why = nullptr) {
 void release(mega_pipeline* p) {
   for (void* b : p->w)
     if (b) (void)hipFree(b);
+  if (p->zf_work) (void)hipFree(p->zf_work);
   for (hipEvent_t e : p->zf_ev)
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : p->eq_ev)
@@ -123,6 +126,13 @@ int mega_pipeline_create(const mega_equalize_desc* shape, uint32_t depth, mega_s
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->zf_ev[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->eq_ev[k], hipEventDisableTiming);
   }
+  // one ZF workspace: the ZF launches follow each other on the ZF stream
+  const uint64_t zb = mega::detail::zf_wide_work_bytes(a.U, a.B, a.slots * a.G);
+  if (e == hipSuccess && p->form == MEGA_FORM_FP32 && zb > 0 &&
+      mega::detail::zf_wide_pays(a.U, a.B, a.slots * a.G)) {
+    e = hipMalloc(&p->zf_work, zb);
+    if (e == hipSuccess) p->zf_work_bytes = zb;
+  }
   if (e != hipSuccess) {
     release(p);
     return hip_err(e, "mega_pipeline_create");
@@ -147,7 +157,8 @@ int mega_pipeline_push(mega_pipeline_t p, const void* csi, const void* y, void* 
           ? mega::detail::launch_zf_mixed(p->a.U, p->a.B, p->a.slots * p->a.G, in, nullptr,
                                           p->w[k], nullptr, nullptr, p->zs, &why)
           : mega::detail::launch_zf(p->a.U, p->a.B, p->a.slots * p->a.G, in,
-                                    static_cast<float2*>(p->w[k]), nullptr, p->zs, &why);
+                                    static_cast<float2*>(p->w[k]), nullptr, p->zs, &why,
+                                    p->zf_work, p->zf_work_bytes, false);
   if (e != hipSuccess) return hip_err(e, "mega_pipeline_push", why);
   e = hipEventRecord(p->zf_ev[k], p->zs);
   if (e != hipSuccess) return hip_err(e, "mega_pipeline_push");

@@ -95,11 +95,23 @@ __global__ __launch_bounds__(256) void scale_absmax_kernel(
 
 hipError_t launch_zf(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
                      float2* out, int32_t* info, hipStream_t st,
-                     const char** why) {
+                     const char** why, void* work, uint64_t work_bytes, bool may_alloc) {
   if (batch == 0) return hipSuccess;
   if (batch > 0x7fffffffu) {
     *why = "batch too large";
     return hipErrorInvalidValue;
+  }
+  // few wide groups: the three-kernel form when a workspace is at hand
+  // (MEGA_ZF_WIDE = 1 never, 2 wherever the shape allows)
+  const int wide = tune(kZfWide);
+  const uint64_t need = zf_wide_work_bytes(U, B, batch);
+  if (need > 0 && wide != 1 && (wide == 2 || zf_wide_pays(U, B, batch))) {
+    void* ws = work && work_bytes >= need ? work : nullptr;
+    if (!ws && !work && may_alloc && stream_workspace(st, need, &ws) != hipSuccess) {
+      ws = nullptr;
+      (void)hipGetLastError();  // a refused workspace leaves the one-workgroup form
+    }
+    if (ws) return launch_zf_wide(U, B, batch, in, out, info, ws, st, why);
   }
   if (U <= 16) return launch_zf_ut<1>(U, B, batch, in, out, info, st);
   if (U <= 32) return launch_zf_ut<2>(U, B, batch, in, out, info, st);

@@ -99,6 +99,55 @@ __device__ __forceinline__ void stage_a(const float2* A, uint32_t U, uint32_t B,
   }
 }
 
+// One Gram item from global A (ld = U): tile (mi, mj) over antenna chunk ch.
+// All loads of the chunk are issued up front (gram_item_fetch) with clamped,
+// always valid addresses and no branches; rows past U carry clamped data (the
+// caller clears them), antennas past B are multiplied by 0 (the steps past
+// the last antenna then add exact zeros). gram_item_mfma accumulates the
+// four real products of every step; the item's value is
+// (gr1 + gr2, gi1 + gi2) per result register.
+struct GramBuf {
+  float2 ai[kGramChunk], aj[kGramChunk];
+};
+
+__device__ __forceinline__ void gram_item_fetch(const float2* A, uint32_t U, uint32_t B, int mi,
+                                                int mj, uint32_t ch, GramBuf& L) {
+  const int lane = threadIdx.x & 63, i16 = lane & 15, q4 = lane >> 4;
+  const uint32_t ui = min(16u * mi + i16, U - 1), uj = min(16u * mj + i16, U - 1);
+#pragma unroll
+  for (uint32_t s = 0; s < kGramChunk; ++s) {
+    const uint32_t b = min(4 * (ch * kGramChunk + s) + q4, B - 1);
+    L.ai[s] = A[ui + (uint64_t)U * b];
+    L.aj[s] = A[uj + (uint64_t)U * b];
+  }
+}
+
+__device__ __forceinline__ void gram_item_mfma(const GramBuf& L, uint32_t B, uint32_t ch,
+                                               v4f& gr1, v4f& gr2, v4f& gi1, v4f& gi2) {
+  const int q4 = (threadIdx.x & 63) >> 4;
+  __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of the MFMAs
+#pragma unroll
+  for (uint32_t s = 0; s < kGramChunk; ++s) {
+    const float keep = 4 * (ch * kGramChunk + s) + q4 < B ? 1.f : 0.f;
+    const float2 ai = make_float2(L.ai[s].x * keep, L.ai[s].y * keep), aj = L.aj[s];
+    gr1 = mfma4(ai.x, aj.x, gr1);
+    gr2 = mfma4(ai.y, aj.y, gr2);
+    gi1 = mfma4(ai.y, aj.x, gi1);
+    gi2 = mfma4(-ai.x, aj.y, gi2);
+  }
+}
+
+// Entry (lane l, register r) of Gram tile (mi, mj) into G (rows of LDG) and
+// its Hermitian mirror; rows or columns past U are written as 0.
+template <int LDG>
+__device__ __forceinline__ void gram_put_tile(float2* G, uint32_t U, int mi, int mj, int l,
+                                              int r, float2 v) {
+  const int row = 16 * mi + 4 * (l >> 4) + r, col = 16 * mj + (l & 15);
+  if (row >= (int)U || col >= (int)U) v = make_float2(0.f, 0.f);
+  G[row * LDG + col] = v;
+  if (mi != mj) G[col * LDG + row] = make_float2(v.x, -v.y);
+}
+
 __device__ __forceinline__ void tile_of(int tile, int& mi, int& mj) {
   mi = 0;
   mj = tile;
@@ -241,33 +290,8 @@ __device__ __forceinline__ void zf_gram(const float2* A, uint32_t U, uint32_t B,
   auto aval = [&](uint32_t u, uint32_t b) -> float2 {
     return (u < U && b < B) ? Asrc[u + (uint64_t)lda * b] : make_float2(0.f, 0.f);
   };
-  // From global memory (no staging): a chunk's loads are all issued up
-  // front (gram_fetch), with clamped (always valid) addresses and no
-  // branches. Rows past U carry clamped data and are cleared in put_tile;
-  // antennas past B are multiplied by 0 in gram_steps (the steps past the
-  // last antenna then add exact zeros, so the sum equals the one over the
-  // valid steps only).
-  struct GramBuf {
-    float2 ai[kGramChunk], aj[kGramChunk];
-  };
+  // From global memory (no staging): gram_item_fetch / gram_item_mfma.
   GramBuf gbuf[2];
-  auto gram_fetch = [&](int mi, int mj, uint32_t ch, GramBuf& L) {
-    const uint32_t ui = min(16u * mi + i16, U - 1), uj = min(16u * mj + i16, U - 1);
-#pragma unroll
-    for (uint32_t s = 0; s < kGramChunk; ++s) {
-      const uint32_t b = min(4 * (ch * kGramChunk + s) + q4, B - 1);
-      L.ai[s] = A[ui + (uint64_t)U * b];
-      L.aj[s] = A[uj + (uint64_t)U * b];
-    }
-  };
-  auto gram_steps = [&](const GramBuf& L, uint32_t ch, auto&& step) {
-    __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of the MFMAs
-#pragma unroll
-    for (uint32_t s = 0; s < kGramChunk; ++s) {
-      const float keep = 4 * (ch * kGramChunk + s) + q4 < B ? 1.f : 0.f;
-      step(make_float2(L.ai[s].x * keep, L.ai[s].y * keep), L.aj[s]);
-    }
-  };
   // Gram of tile (mi, mj) over antenna chunk ch: this lane's 4 entries.
   auto gram_chunk = [&](int mi, int mj, uint32_t ch, float2 (&out)[4]) {
     const uint32_t s0 = ch * kGramChunk;
@@ -287,17 +311,14 @@ __device__ __forceinline__ void zf_gram(const float2* A, uint32_t U, uint32_t B,
         step(ai, (mi == mj) ? ai : aval(uj, b));
       }
     } else {
-      gram_fetch(mi, mj, ch, gbuf[0]);
-      gram_steps(gbuf[0], ch, step);
+      gram_item_fetch(A, U, B, mi, mj, ch, gbuf[0]);
+      gram_item_mfma(gbuf[0], B, ch, gr1, gr2, gi1, gi2);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[r] = make_float2(gr1[r] + gr2[r], gi1[r] + gi2[r]);
   };
   auto put_tile = [&](int mi, int mj, int l, int r, float2 v) {
-    const int row = 16 * mi + 4 * (l >> 4) + r, col = 16 * mj + (l & 15);
-    if (row >= (int)U || col >= (int)U) v = make_float2(0.f, 0.f);
-    sG[row * LDG + col] = v;
-    if (mi != mj) sG[col * LDG + row] = make_float2(v.x, -v.y);
+    gram_put_tile<LDG>(sG, U, mi, mj, l, r, v);
   };
 
   // ---- 1. Gram ---------------------------------------------------------------
@@ -338,7 +359,7 @@ __device__ __forceinline__ void zf_gram(const float2* A, uint32_t U, uint32_t B,
       int mi, mj;
       if ((uint32_t)wave < nitems) {
         item_tile(wave, mi, mj);
-        gram_fetch(mi, mj, wave % nch, gbuf[0]);
+        gram_item_fetch(A, U, B, mi, mj, wave % nch, gbuf[0]);
       }
       for (uint32_t it0 = wave; it0 < nitems; it0 += 2 * NW) {
 #pragma unroll
@@ -347,15 +368,10 @@ __device__ __forceinline__ void zf_gram(const float2* A, uint32_t U, uint32_t B,
           if (item >= nitems) break;
           if (item + NW < nitems) {
             item_tile(item + NW, mi, mj);
-            gram_fetch(mi, mj, (item + NW) % nch, gbuf[r ^ 1]);
+            gram_item_fetch(A, U, B, mi, mj, (item + NW) % nch, gbuf[r ^ 1]);
           }
           v4f gr1 = {0, 0, 0, 0}, gr2 = {0, 0, 0, 0}, gi1 = {0, 0, 0, 0}, gi2 = {0, 0, 0, 0};
-          gram_steps(gbuf[r], item % nch, [&](float2 ai, float2 aj) {
-            gr1 = mfma4(ai.x, aj.x, gr1);
-            gr2 = mfma4(ai.y, aj.y, gr2);
-            gi1 = mfma4(ai.y, aj.x, gi1);
-            gi2 = mfma4(-ai.x, aj.y, gi2);
-          });
+          gram_item_mfma(gbuf[r], B, item % nch, gr1, gr2, gi1, gi2);
           put_part(item, gr1, gr2, gi1, gi2);
         }
       }

@@ -4,6 +4,7 @@
     python -m megastation_amd.profiling traffic <FETCH_SIZE dir> <WRITE_SIZE dir> \
         [--copy-bytes N] [--json out.json]
     python -m megastation_amd.profiling counters <dir with *_counter_collection.csv>
+    python -m megastation_amd.profiling sqtable <dir of an SQ counter pass>
 
 `traffic` turns per-dispatch FETCH_SIZE / WRITE_SIZE counters (KiB) into HBM
 bytes per launch. Each counter is scaled by what it reads for a plain copy of
@@ -76,6 +77,42 @@ def traffic(fetch_dir: str, write_dir: str, copy_bytes: int) -> dict:
     return res
 
 
+# MI355X: 8 XCDs, 256 CUs, 4 SIMDs per CU, 2.4 GHz peak engine clock.
+XCDS, SIMDS, CLOCK_GHZ = 8, 1024, 2.4
+
+
+def sq_table(d: str, prefixes: tuple) -> str:
+    """Per-kernel medians of one SQ counter pass and the ratios read from them:
+      kernel cycles  = GRBM_GUI_ACTIVE / 8 (the counter is summed over the
+                       eight XCDs; / 2.4 GHz gives the duration)
+      MFMA busy      = SQ_VALU_MFMA_BUSY_CYCLES / (1024 SIMDs * kernel cycles)
+                       (the counter sums matrix-core busy cycles over SIMDs)
+      VALU active    = SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES (per wave-cycle)
+      wait           = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES (share of wave time
+                       spent waiting for an operand: memory, LDS, MFMA result)
+      waves resident = 4 * SQ_WAVE_CYCLES / kernel cycles / 256 (per CU,
+                       average; the SQ wave counters count quad-cycles)
+      LDS conflicts  = SQ_LDS_BANK_CONFLICT / SQ_INSTS_LDS (cycles per LDS
+                       instruction)"""
+    agg = counter_by_kernel(d)
+    kernels = sorted({k for (k, _) in agg if k.startswith(prefixes)})
+    med = lambda k, c: statistics.median(agg[(k, c)]) if (k, c) in agg else float("nan")
+    rows = ["| kernel | dispatches | kernel us | MFMA busy | VALU active | wait | "
+            "waves/CU | LDS conflict cyc/instr |", "|---|---|---|---|---|---|---|---|"]
+    for k in kernels:
+        gui = med(k, "GRBM_GUI_ACTIVE") / XCDS
+        wc = med(k, "SQ_WAVE_CYCLES")
+        lds = med(k, "SQ_INSTS_LDS")
+        conf = med(k, "SQ_LDS_BANK_CONFLICT")
+        rows.append(
+            f"| {k} | {len(agg[(k, 'SQ_WAVE_CYCLES')])} | {gui / CLOCK_GHZ / 1e3:.1f} "
+            f"| {med(k, 'SQ_VALU_MFMA_BUSY_CYCLES') / (SIMDS * gui):.3f} "
+            f"| {med(k, 'SQ_ACTIVE_INST_VALU') / wc:.3f} "
+            f"| {med(k, 'SQ_WAIT_INST_ANY') / wc:.3f} | {4 * wc / gui / 256:.2f} "
+            f"| {(conf / lds) if lds else 0.0:.2f} |")
+    return "\n".join(rows)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -88,6 +125,9 @@ def main():
     t.add_argument("--json")
     c = sub.add_parser("counters")
     c.add_argument("dir")
+    q = sub.add_parser("sqtable")
+    q.add_argument("dir")
+    q.add_argument("--prefix", default="equalize,zf,detect,fft,precode,ldpc")
     a = ap.parse_args()
     if a.cmd == "stats":
         print("| kernel | calls | avg us | min us | max us | % time |")
@@ -101,6 +141,8 @@ def main():
             with open(a.json, "w") as fh:
                 json.dump(res, fh, indent=1)
         print(json.dumps(res, indent=1))
+    elif a.cmd == "sqtable":
+        print(sq_table(a.dir, tuple(a.prefix.split(","))))
     else:
         for (k, cn), vals in sorted(counter_by_kernel(a.dir).items()):
             print(f"{k:40s} {cn:28s} n={len(vals)} median={statistics.median(vals):.6g}")
