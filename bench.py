@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--workload", default="c3_256x32",
                     choices=sorted(WORKLOADS) + ["c4_40cells", "c5_learned",
                                                  "d3_precode_256x32", "u3_fft_256x32",
-                                                 "l3_ldpc_256x32"])
+                                                 "l3_ldpc_256x32", "uc3_uplink_256x32"])
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="approximate CPU time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -142,6 +142,8 @@ def main():
         return main_fft(args)
     if args.workload == "l3_ldpc_256x32":
         return main_ldpc(args)
+    if args.workload == "uc3_uplink_256x32":
+        return main_uplink(args)
 
     if args.copy_calibration:
         import torch
@@ -888,7 +890,7 @@ def main_fft(args):
         "roofline": {"bound": "hbm", "kernel": "fft_kernel (uplink, 13 symbols)",
                      "achieved": up_bytes / (up_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": up_bytes / (up_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel") or
+                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel<short, 0>") or
                                  {}).get("traffic_bytes"),
                      "algorithmic_bytes": up_bytes, "avg_ms": up_ms},
         "forms": {"pilot_ms": pil_ms, "uplink_ms": up_ms},
@@ -905,6 +907,178 @@ def main_fft(args):
                                   "cores": 1, "kind": "port",
                                   "sample": "1 of 13 data symbols (256 antennas), median of 3, "
                                             "numpy FFT (the oracle)"}
+    print(json.dumps(result), flush=True)
+
+
+def uplink_slot_samples(seed=1234):
+    """One 256 x 32 slot as received int16 samples: 3 pilot symbols (pilot
+    symbol s carries users 12 s .. 12 s + 11 on the subcarriers of every
+    group, the channel divided by the conjugate pilot sign, as the reference's
+    pilot storer expects) and 13 data symbols (64-QAM over a Kronecker
+    channel at 25 dB), bins 410.. of a 4096-point spectrum, taken to the time
+    domain with numpy and scaled to 40 % of int16 full scale. Returns
+    (pilot samples [3][B][4096][2], data samples [13][B][4096][2], the slot
+    (synth.Slot), the common scale of the FFT outputs)."""
+    from megastation_amd import synth
+    import megastation_amd as m
+    B, U, ca, N, sg, S, P, start, Q = 256, 32, 4096, 3276, 12, 13, 3, 410, 6
+    G = (N + sg - 1) // sg
+    rng = np.random.Generator(np.random.Philox(seed))
+    sl = synth.make_slot(rng, U, B, N, sg, S, Q, 25.0, "kronecker")
+    p = m.pilot_sign(N)
+    cols = (np.arange(N) + start + ca // 2) % ca
+    X = np.zeros((P + S, B, ca), np.complex64)
+    for s_ in range(P):
+        u = s_ * sg + np.arange(N) % sg
+        ok = u < U
+        i = np.arange(N)[ok]
+        X[s_][:, cols[i]] = (sl.H[i // sg, :, u[ok]] / np.conj(p[i])[:, None]).T
+    X[P:, :, cols] = sl.y
+    xt = np.fft.ifft(X, axis=2)
+    gain = 0.4 / max(np.abs(xt.real).max(), np.abs(xt.imag).max())
+    iq = np.empty((P + S, B, ca, 2), np.int16)
+    iq[..., 0] = np.round(xt.real * (32768 * gain))
+    iq[..., 1] = np.round(xt.imag * (32768 * gain))
+    return iq[:P], iq[P:], sl, gain
+
+
+def main_uplink(args):
+    """The uplink slot of a 256 x 32 cell from received samples to LLRs: 3
+    pilot symbols through mega_pilot_fft (CSI, A = H^H), mega_zf (FP32 W),
+    then the 13 data symbols through the uplink data path -- FFT and
+    equalize + demod (64-QAM, LLRs in the standalone layout). Forms: staged
+    (mega_uplink_fft writes the slot's y [13][256][3276] into that slot's
+    own buffer, then one mega_equalize_slots) and fused (mega_uplink_run: the
+    same two launches through the handle's one reused y buffer, which stays
+    in the Infinity Cache; LLRs equal the staged form's bit for bit). value =
+    detected symbols/s of the faster FP32 form; EVM of the chain against FP64
+    ZF with the true channel. Four rotating buffer sets of samples, CSI, W, y
+    (staged) and LLRs."""
+    import torch
+    import megastation_amd as m
+    from megastation_amd import synth
+    B, U, ca, N, sg, S, P, start, Q = 256, 32, 4096, 3276, 12, 13, 3, 410, 6
+    G = (N + sg - 1) // sg
+    dev = torch.device("cuda:0")
+    print("[bench] uplink_256x32: building the slot's samples", file=sys.stderr, flush=True)
+    pil, dat, sl, scale = uplink_slot_samples()
+    NR = 4
+    PL = [torch.from_numpy(pil).to(dev) for _ in range(NR)]
+    D = [torch.from_numpy(dat).to(dev) for _ in range(NR)]
+    C = [torch.zeros(G * B * U, dtype=torch.complex64, device=dev) for _ in range(NR)]
+    W = [torch.empty_like(C[0]) for _ in range(NR)]
+    Y = [torch.empty(S * B * N, dtype=torch.complex64, device=dev) for _ in range(NR)]
+    lay = synth.standalone_layout(N, Q)
+    L = [torch.empty(S * U * lay.ue_stride, dtype=torch.int16, device=dev) for _ in range(NR)]
+    psign = torch.from_numpy(m.pilot_sign(N)).to(dev)
+    stream = torch.cuda.current_stream()
+    fk = dict(ofdm_start=start, ofdm_ca=ca, ofdm_num=N, bs_num=B, stream=stream)
+    eq = dict(num_ue=U, num_bs=B, num_sc=N, sc_group=sg, num_syms=S, num_slots=1, mod_order=Q,
+              ue_stride=lay.ue_stride, cb_size=lay.cb_size, cb_stride=lay.cb_stride,
+              zeros=lay.zeros, valid_bits=lay.valid_bits, csi_slot_stride=G * B * U,
+              y_sym_stride=B * N, y_slot_stride=S * B * N, llr_sym_stride=U * lay.ue_stride,
+              llr_slot_stride=S * U * lay.ue_stride)
+    up = m.UplinkDetect(ofdm_start=start, ofdm_ca=ca, ofdm_num=N, bs_num=B, num_syms=S,
+                        num_ue=U, sc_group=sg, mod_order=Q, ue_stride=lay.ue_stride,
+                        cb_size=lay.cb_size, cb_stride=lay.cb_stride, zeros=lay.zeros,
+                        valid_bits=lay.valid_bits)
+
+    def front(j):
+        m.pilot_fft(PL[j], C[j], psign, ue_num=U, sc_group=sg, num_syms=P, **fk)
+        m.zf(C[j], W[j], B, U, G, stream=stream)
+
+    def staged(j, evs=None, xhat=None):
+        front(j)
+        if evs:
+            evs[0].record(stream)
+        m.uplink_fft(D[j], Y[j], num_syms=S, **fk)
+        if evs:
+            evs[1].record(stream)
+        m.equalize_slots(W[j], Y[j], L[j], xhat=xhat, stream=stream, **eq)
+
+    def fused(j, evs=None, xhat=None):
+        front(j)
+        up.run(D[j], W[j], L[j], xhat=xhat, stream=stream)
+
+    def timed(step):
+        for j in range(args.warmup):
+            step(j % NR)
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)]
+               for _ in range(args.steps)]
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for j in range(args.steps):
+            step(j % NR, evs[j])
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / args.steps, evs
+
+    ms_staged, evs = timed(staged)
+    fft_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
+    ms_fused, _ = timed(fused)
+    print(f"[bench] uplink_256x32: staged {ms_staged:.4f} ms, fused {ms_fused:.4f} ms, "
+          f"data FFT {fft_ms:.4f} ms", file=sys.stderr, flush=True)
+    # LLRs of the two forms, and the chain's EVM
+    xh = torch.zeros(S * U * N, dtype=torch.complex64, device=dev)
+    staged(0, xhat=xh)
+    torch.cuda.synchronize()
+    l_staged = L[0].clone()
+    fused(0, xhat=xh)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(l_staged, L[0]))
+    x_chain = xh.cpu().numpy().reshape(S, U, N)
+    cfg = dict(U=U, B=B, N=N, sg=sg, S=S)
+    x64 = fp64_xhat(cfg, torch.from_numpy(sl.csi[None]), torch.from_numpy(sl.y[None]))[0]
+    evm_gpu, evm64 = synth.evm_db(x_chain, sl.x), synth.evm_db(x64, sl.x)
+    head = "fused" if ms_fused <= ms_staged else "staged"
+    ms = min(ms_fused, ms_staged)
+    symbols = U * N * S
+    fft_bytes = S * B * (4 * ca + 8 * N)
+    result = {
+        "metric": "detected symbols/s from received samples (pilot FFT + ZF + data FFT + "
+                  "equalize + demod), 1 GPU, at EVM within 0.1 dB of FP64",
+        "value": symbols / (ms * 1e-3), "unit": "symbols/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (int16 samples)", "data": "synthetic",
+        "config": {"workload": "uc3_uplink_256x32", "antennas": B, "users": U, "fft_size": ca,
+                   "subcarriers": N, "sc_group": sg, "data_symbols": S, "pilot_symbols": P,
+                   "qam": 1 << Q, "snr_db": 25.0, "channel": "kronecker", "form": head,
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": "fft_kernel (uplink, 13 symbols, staged form)",
+                     "achieved": fft_bytes / (fft_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": fft_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel<short, 0>") or
+                                 {}).get("traffic_bytes"),
+                     "algorithmic_bytes": fft_bytes, "avg_ms": fft_ms},
+        "forms": {"staged": {"ms_per_step": ms_staged, "symbols_per_s": symbols / (ms_staged * 1e-3)},
+                  "fused": {"ms_per_step": ms_fused, "symbols_per_s": symbols / (ms_fused * 1e-3),
+                            "y_buffer_bytes": S * B * N * 8,
+                            "llr_bit_identical_to_staged": same},
+                  "fused_gain": ms_staged / ms_fused},
+        "evm": {"gpu_db": evm_gpu, "fp64_db": evm64, "delta_db": evm_gpu - evm64,
+                "within_0p1_db": abs(evm_gpu - evm64) < 0.1},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import fft as offt
+        iq = dat[0].reshape(-1)
+        ts = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            offt.uplink_fft(iq, start, ca, N, B)
+            ts.append(time.perf_counter() - c0)
+        det = cpu_baseline(dict(cfg, slots=1, Q=Q), lay,
+                           torch.from_numpy(np.conj(sl.H)[None].astype(np.complex64)),
+                           torch.from_numpy(sl.y[None]), None, args.cpu_seconds)
+        t_slot = (S + P) * statistics.median(ts) + symbols / det["value"]
+        result["cpu_baseline"] = {
+            "value": symbols / t_slot, "unit": "symbols/s", "cores": det["cores"],
+            "kind": "port",
+            "sample": f"numpy FFT (the oracle) of one symbol of 256 antennas (median of 3, "
+                      f"1 thread) times 16 symbols, plus the OpenMP detection restatement of "
+                      f"the slot ({det['cores']} threads)"}
+    up.close()
     print(json.dumps(result), flush=True)
 
 

@@ -222,6 +222,36 @@ int mega_downlink_ifft(const mega_fft_desc *desc, mega_stream_t stream);
  * host_out receives ofdm_num (re, im) float pairs. Synchronous, no GPU. */
 int mega_pilot_sign(uint32_t ofdm_num, float *host_out);
 
+/* ---- Uplink data path: samples -> FFT -> equalize + demod -------------------
+ * One slot's uplink symbols from received samples to LLRs through y buffers
+ * the handle owns. By default (chunk_syms 0, or >= num_syms) one FFT launch
+ * (mega_uplink_fft's kernel) and one equalize + demod launch
+ * (mega_equalize_slots' kernels) on `stream`, through one slot of y reused
+ * every run (it stays in the Infinity Cache between the two). chunk_syms =
+ * K < S runs the symbols in chunks of K through a ring of two chunk buffers:
+ * the FFT of chunk c on `stream`, the equalize of chunk c on the handle's
+ * own stream, the FFT of chunk c + 2 after the equalize of chunk c, so one
+ * chunk's FFT runs beside the previous chunk's equalize and y needs only
+ * 2 K B N complex (slower at 256 x 32: both kernels already fill the GPU;
+ * MEGA_UPLINK_STREAMS=1 keeps the chunks on `stream`). The reference runs
+ * UplinkFFT, Equalize::equalize and the decoder per symbol with a y buffer
+ * per symbol (src/baseband/process.cc:175-186). fft: ofdm_start, ofdm_ca,
+ * ofdm_num (= shape->num_sc), bs_num (= num_bs), num_syms (= num_syms),
+ * quant_bits, in_sym_stride (its buffers are ignored); shape: as
+ * mega_equalize_slots for one slot (num_slots = 1; form FP32 with W in the
+ * CSI layout, or MIXED with mega_zf_split's planes), its buffers and y
+ * strides ignored. mega_uplink_run(samples = [S][B][ofdm_ca] integer pairs,
+ * w, llr, xhat optional [S][U][N]) is asynchronous on `stream`, which ends
+ * after the last equalize; LLRs and x_hat equal mega_uplink_fft of the slot
+ * followed by mega_equalize_slots, bit for bit in the FP32 form. One run at
+ * a time per handle (runs on one stream follow each other). */
+typedef struct mega_uplink *mega_uplink_t;
+int mega_uplink_create(const mega_fft_desc *fft, const mega_equalize_desc *shape,
+                       uint32_t chunk_syms, mega_uplink_t *out);
+int mega_uplink_run(mega_uplink_t uplink, const void *samples, const void *w, void *llr,
+                    void *xhat, mega_stream_t stream);
+int mega_uplink_destroy(mega_uplink_t uplink);
+
 /* ---- Downlink precode ----------------------------------------------------- */
 
 /* Replaces the raw overload Precode::precode(problem_size_row = B,

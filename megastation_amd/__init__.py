@@ -156,6 +156,10 @@ def lib() -> ctypes.CDLL:
         "mega_demod_host": ([P, u64, u8, P], i32),
         "mega_precode": ([u64, u64, u64, P, P, P, u64, u64, u64, P], i32),
         "mega_uplink_fft": ([ctypes.POINTER(FftDesc), P], i32),
+        "mega_uplink_create": ([ctypes.POINTER(FftDesc), ctypes.POINTER(EqualizeDesc), u32,
+                                ctypes.POINTER(ctypes.c_void_p)], i32),
+        "mega_uplink_run": ([P, P, P, P, P, P], i32),
+        "mega_uplink_destroy": ([P], i32),
         "mega_pilot_fft": ([ctypes.POINTER(FftDesc), u32, u32, u32, P, P], i32),
         "mega_pilot_sign": ([u32, P], i32),
         "mega_downlink_ifft": ([ctypes.POINTER(FftDesc), P], i32),
@@ -355,6 +359,45 @@ def uplink_fft(iq, y, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms: int = 
     d = _fft_desc(iq, y, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms, in_sym_stride,
                   out_sym_stride)
     _check(lib().mega_uplink_fft(ctypes.byref(d), _stream(stream)))
+
+
+class UplinkDetect:
+    """The uplink data path of one slot (mega_uplink_*): received samples
+    [S][B][ofdm_ca] (int16 or int8 pairs) -> FFT -> equalize + demod -> LLRs
+    through y buffers the handle owns: one FFT and one equalize launch over
+    a reused slot of y (chunk_syms 0), or chunks of chunk_syms symbols
+    through a two-chunk ring, the FFT of one chunk beside the equalize of the
+    previous. The equalize shape is mega_equalize_slots' for one slot; form
+    FP32 takes W in the CSI layout, MIXED mega_zf_split's planes."""
+
+    def __init__(self, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms, quant_bits=16,
+                 in_sym_stride=0, num_ue, sc_group, mod_order, ue_stride, cb_size, cb_stride,
+                 zeros, valid_bits, llr_sym_stride=0, form: int = FORM_FP32,
+                 chunk_syms: int = 0):
+        f = FftDesc(ofdm_start, ofdm_ca, ofdm_num, bs_num, num_syms, quant_bits, None,
+                    in_sym_stride, None, 0)
+        e = EqualizeDesc(num_ue, bs_num, ofdm_num, sc_group, num_syms, 1, mod_order, form,
+                         ue_stride, cb_size, cb_stride, zeros, valid_bits, None, 0, None, 0, 0,
+                         None, llr_sym_stride, 0, None)
+        h = ctypes.c_void_p()
+        _check(lib().mega_uplink_create(ctypes.byref(f), ctypes.byref(e), chunk_syms,
+                                        ctypes.byref(h)))
+        self._h = h
+
+    def run(self, samples, w, llr, xhat=None, stream=None):
+        _check(lib().mega_uplink_run(self._h, _ptr(samples), _ptr(w), _ptr(llr), _ptr(xhat),
+                                     _stream(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mega_uplink_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def pilot_fft(iq, csi, pilot, *, ofdm_start, ofdm_ca, ofdm_num, bs_num, ue_num, sc_group,

@@ -21,7 +21,7 @@ LIB = os.path.join(LIBDIR, "libmegadetect.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["capi.cpp", "equalize.hip", "zf.hip", "detect.hip", "zf_mixed.hip",
-           "equalize_mixed.hip", "batch.hip", "batch.cpp", "mlp.hip", "learned.hip", "pipeline.cpp", "precode.hip", "fft.hip", "tuning.cpp", "equalize_wide.hip", "ldpc.hip", "zf_wide.hip", "workspace.cpp"]
+           "equalize_mixed.hip", "batch.hip", "batch.cpp", "mlp.hip", "learned.hip", "pipeline.cpp", "precode.hip", "fft.hip", "tuning.cpp", "equalize_wide.hip", "ldpc.hip", "zf_wide.hip", "workspace.cpp", "uplink.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -38,6 +38,7 @@ constexpr TuneSpec kSpecs[kTuneCount] = {
     {"MEGA_EQ_WTC", 0, 1, 1 << 20},  // FP32 wide form: column tiles per workgroup
     {"MEGA_ZF_NTH", 0, 256, 512},   // FP32 ZF, 17-32 users: threads per group
     {"MEGA_ZF_WIDE", 0, 1, 2},      // FP32 ZF: 1 one workgroup per group, 2 three kernels
+    {"MEGA_UPLINK_STREAMS", 0, 1, 2},  // uplink data path: 1 one stream, 2 two streams
 };
 
 std::atomic<int> g_vals[kTuneCount];
