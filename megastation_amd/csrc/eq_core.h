@@ -419,7 +419,8 @@ hipError_t fill_eq_params(const EqualizeArgs& a, EqParams& p, const char** why);
 // waves of a workgroup, W slices in registers, three-product complex MFMAs.
 // Shapes it takes: up to 32 users, 33 to 256 antennas.
 bool eq_wide_ok(uint32_t U, uint32_t B);
-hipError_t launch_equalize_wide(const EqParams& p, uint32_t tc_override, hipStream_t st,
+hipError_t launch_equalize_wide(const EqParams& p, uint32_t tc_override, uint32_t nw_override,
+                                uint32_t persistent, hipStream_t st,
                                 const char** why);
 
 // Host-side tile plan shared by the equalize and detect launchers.

@@ -174,7 +174,8 @@ hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t st,
   const int mode = tune(kEqMode);
   if (eq_wide_ok(a.U, a.B) &&
       (mode == 3 || (mode == 0 && a.U > 16 && a.B >= 128)))
-    return launch_equalize_wide(p, (uint32_t)tune(kEqWtc), st, why);
+    return launch_equalize_wide(p, (uint32_t)tune(kEqWtc), (uint32_t)tune(kEqWnw),
+                                (uint32_t)tune(kEqWp), st, why);
   // Tile plan; MEGA_EQ_KS=8|16 overrides the antenna step count for sweeps.
   const EqPlan pl = plan_equalize(a.U, a.B, (uint32_t)tune(kEqKs));
   // W held in registers while one wave covers all antennas; otherwise W is

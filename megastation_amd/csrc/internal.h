@@ -146,7 +146,7 @@ hipError_t launch_scale_absmax(uint32_t U, uint32_t B, uint32_t batch,
 // the launcher's own choice except for kDetPf (default 1).
 enum TuneKey {
   kEqKs, kEqMode, kEqSch, kEqSmt, kEqTpw, kEqXmt, kEqXreg, kEqXt, kEqXks, kPrecAcc, kDetPf,
-  kEqWtc, kZfNth, kZfWide, kUplinkStreams,
+  kEqWtc, kZfNth, kZfWide, kUplinkStreams, kEqWnw, kEqWp,
   kTuneCount
 };
 int tune(TuneKey k);

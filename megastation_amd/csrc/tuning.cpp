@@ -39,6 +39,8 @@ constexpr TuneSpec kSpecs[kTuneCount] = {
     {"MEGA_ZF_NTH", 0, 256, 512},   // FP32 ZF, 17-32 users: threads per group
     {"MEGA_ZF_WIDE", 0, 1, 2},      // FP32 ZF: 1 one workgroup per group, 2 three kernels
     {"MEGA_UPLINK_STREAMS", 0, 1, 2},  // uplink data path: 1 one stream, 2 two streams
+    {"MEGA_EQ_WNW", 0, 4, 8},       // FP32 wide form: waves per workgroup (4 or 8)
+    {"MEGA_EQ_WP", 0, 1, 8},        // FP32 wide form: persistent, k * 256 workgroups
 };
 
 std::atomic<int> g_vals[kTuneCount];

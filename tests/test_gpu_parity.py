@@ -152,12 +152,19 @@ EQ_CASES = [
 # MEGA_EQ_MODE: 0 = the launcher's choice, 1 = W streamed from memory,
 # 2 = W held in registers (antennas split over waves when B is wide),
 # 3 = the wide form (antennas split over a workgroup's waves, three-product
-# MFMAs) where it applies (up to 32 users, 33 to 256 antennas)
-@pytest.mark.parametrize("mode", ["0", "1", "2", "3"])
+# MFMAs) where it applies (up to 32 users, 33 to 256 antennas); 3w8 = the
+# wide form on eight waves per workgroup (MEGA_EQ_WNW=8); 3p = its persistent
+# form (MEGA_EQ_WP=2: 512 workgroups over balanced tile ranges)
+@pytest.mark.parametrize("mode", ["0", "1", "2", "3", "3w8", "3p"])
 @pytest.mark.parametrize("U,B,N,sg,Q,layout", EQ_CASES)
 def test_equalize_matches_oracle(U, B, N, sg, Q, layout, mode, tuning):
     print(f"case: equalize U={U} B={B} N={N} sg={sg} Q={Q} {layout} mode={mode}", flush=True)
-    tuning(MEGA_EQ_MODE=mode)
+    if mode == "3w8":
+        tuning(MEGA_EQ_MODE=3, MEGA_EQ_WNW=8)
+    elif mode == "3p":
+        tuning(MEGA_EQ_MODE=3, MEGA_EQ_WP=2)
+    else:
+        tuning(MEGA_EQ_MODE=mode)
     S, slots = 2, 1
     sl = make_slots(U * 7 + N, slots, U, B, N, sg, S, Q)
     lay = synth.ldpc_uplink_layout(N, Q) if layout == "ldpc" else synth.standalone_layout(N, Q)
@@ -184,17 +191,22 @@ def test_equalize_wide_chunking_is_bit_identical(U, B, N, sg, Q, tuning):
     G = (N + sg - 1) // sg
     w = np.stack([oracle.zf(s_.csi, U, B, G, "f64")[0].reshape(G, B, U) for s_ in sl])
     y = np.stack([s_.y for s_ in sl])
-    outs = []
-    for tc in (0, 1, 4, 100):
-        tuning(MEGA_EQ_MODE=3, MEGA_EQ_WTC=tc)
-        outs.append(gpu_equalize_slots(w, y, U, B, N, sg, Q, lay, slots=slots, S=S))
-    for llr, xh in outs[1:]:
-        assert np.array_equal(llr, outs[0][0]) and np.array_equal(xh, outs[0][1])
-    llr, xh = outs[0]
-    for s_ in range(slots):
-        for t_ in range(S):
-            llr64, x64 = oracle_symbol_f64(w[s_], y[s_, t_], U, B, N, sg, Q, lay)
-            check_symbol(llr[s_, t_], xh[s_, t_], llr64, x64, N, sg, Q, lay)
+    for nw in (4, 8):                                 # waves per workgroup
+        outs = []
+        for tc in (0, 1, 4, 100):
+            tuning(MEGA_EQ_MODE=3, MEGA_EQ_WTC=tc, MEGA_EQ_WNW=nw)
+            outs.append(gpu_equalize_slots(w, y, U, B, N, sg, Q, lay, slots=slots, S=S))
+        if nw == 8:                                   # the persistent form: same bits
+            for k in (1, 2, 8):
+                tuning(MEGA_EQ_MODE=3, MEGA_EQ_WP=k)
+                outs.append(gpu_equalize_slots(w, y, U, B, N, sg, Q, lay, slots=slots, S=S))
+        for llr, xh in outs[1:]:
+            assert np.array_equal(llr, outs[0][0]) and np.array_equal(xh, outs[0][1])
+        llr, xh = outs[0]
+        for s_ in range(slots):
+            for t_ in range(S):
+                llr64, x64 = oracle_symbol_f64(w[s_], y[s_, t_], U, B, N, sg, Q, lay)
+                check_symbol(llr[s_, t_], xh[s_, t_], llr64, x64, N, sg, Q, lay)
 
 
 def test_equalize_slots_equals_per_symbol_calls():
