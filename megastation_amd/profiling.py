@@ -123,6 +123,9 @@ def main():
     t.add_argument("write_dir")
     t.add_argument("--copy-bytes", type=int, default=1 << 30)
     t.add_argument("--json")
+    t.add_argument("--workload", help="with --update: the workload key to replace")
+    t.add_argument("--update", help="traffic file (profiles/traffic.json) whose entry for "
+                                    "--workload is replaced by this measurement")
     c = sub.add_parser("counters")
     c.add_argument("dir")
     q = sub.add_parser("sqtable")
@@ -140,6 +143,18 @@ def main():
         if a.json:
             with open(a.json, "w") as fh:
                 json.dump(res, fh, indent=1)
+        if a.update and a.workload:
+            # bench.py reads {workload: {kernel: {traffic_bytes, ...}, _calibration}}
+            db = json.load(open(a.update)) if os.path.exists(a.update) else {}
+            entry = {k: {"traffic_bytes": v["traffic_bytes"], "read_bytes": v["read_bytes"],
+                         "write_bytes": v["write_bytes"], "fetch_kib": v["fetch_kib"],
+                         "write_kib": v["write_kib"]} for k, v in res["kernels"].items()}
+            entry["_calibration"] = {"copy_bytes": res["copy_bytes"],
+                                     "fetch_scale": res["fetch_scale"],
+                                     "write_scale": res["write_scale"]}
+            db[a.workload] = entry
+            with open(a.update, "w") as fh:
+                json.dump(db, fh, indent=1)
         print(json.dumps(res, indent=1))
     elif a.cmd == "sqtable":
         print(sq_table(a.dir, tuple(a.prefix.split(","))))
