@@ -84,6 +84,33 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void equalize_wide(const 
     for (int s = 0; s < KS; ++s) yd[s] = yc[(uint64_t)min(bw + 4 * s + q4, B - 1) * N];
   };
 
+  // outputs of column tile t from partial buffer b: wave w adds up, demods and
+  // stores outputs (m, r) = pi / 4, pi % 4 for pi = w, w + NW, ...
+  auto finish = [&](uint32_t t, int b) {
+    const EqCol col = eq_col<Q>(p, slot, g, t);
+#pragma unroll
+    for (int pi = w; pi < 4 * MT; pi += NW) {
+      const int m = pi / 4, r = pi % 4;
+      float re = red[b][0][m][0][r][lane], im = red[b][0][m][1][r][lane];
+#pragma unroll
+      for (int j = 1; j < NW; ++j) {
+        re += red[b][j][m][0][r][lane];
+        im += red[b][j][m][1][r][lane];
+      }
+      const uint32_t u = 16 * m + 4 * q4 + r;
+      if (col.valid && u < U) {
+        store_llrs<Q>(p, col.rows + u * p.ue_stride, col.ls, re, im);
+        if (col.xrow) col.xrow[(uint64_t)u * N] = make_float2(re, im);
+      }
+    }
+  };
+
+  // Software pipeline: iteration t issues tile t + 1's y loads, then tile
+  // t - 1's outputs, then tile t's MFMAs. Loads and stores share one
+  // completion counter, so the wait for y at the head of an iteration also
+  // waits for every store issued before it; issued ahead of the MFMAs, the
+  // stores complete while the matrix cores work instead of stalling the next
+  // tile.
   float2 ycur[KS], ynext[KS];
   load_y(t0, ynext);
   int buf = 0;
@@ -91,6 +118,7 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void equalize_wide(const 
 #pragma unroll
     for (int s = 0; s < KS; ++s) ycur[s] = ynext[s];
     if (t + 1 < t1) load_y(t + 1, ynext);  // in flight during this tile's MFMAs
+    if (t > t0) finish(t - 1, buf ^ 1);
     __builtin_amdgcn_sched_barrier(0);
     v4f k1[MT], k2[MT], k3[MT];
 #pragma unroll
@@ -114,26 +142,12 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void equalize_wide(const 
         red[buf][w][m][0][r][lane] = k1[m][r] - k3[m][r];
         red[buf][w][m][1][r][lane] = k1[m][r] + k2[m][r];
       }
+    // after this barrier buffer buf holds tile t's partials; buffer buf ^ 1
+    // (tile t - 1's, read above) is free for tile t + 1's
     __syncthreads();
-    // wave w stores outputs (m, r) = pi / 4, pi % 4 for pi = w, w + NW, ...
-    const EqCol col = eq_col<Q>(p, slot, g, t);
-#pragma unroll
-    for (int pi = w; pi < 4 * MT; pi += NW) {
-      const int m = pi / 4, r = pi % 4;
-      float re = red[buf][0][m][0][r][lane], im = red[buf][0][m][1][r][lane];
-#pragma unroll
-      for (int j = 1; j < NW; ++j) {
-        re += red[buf][j][m][0][r][lane];
-        im += red[buf][j][m][1][r][lane];
-      }
-      const uint32_t u = 16 * m + 4 * q4 + r;
-      if (col.valid && u < U) {
-        store_llrs<Q>(p, col.rows + u * p.ue_stride, col.ls, re, im);
-        if (col.xrow) col.xrow[(uint64_t)u * N] = make_float2(re, im);
-      }
-    }
     buf ^= 1;
   }
+  finish(t1 - 1, buf ^ 1);
 }
 
 // Persistent form of the same arithmetic: P workgroups, workgroup i sweeping
@@ -300,7 +314,22 @@ bool eq_wide_ok(uint32_t U, uint32_t B) { return U >= 1 && U <= 32 && B > 32 && 
 // kernel at 2 tiles (four waves): each workgroup's first tile takes 9.4 us
 // (W and y loads in flight, about 1.3 us of MFMAs), later tiles 2.8 us, the
 // grid runs in about three rounds; without the y loads the launch takes
-// 30 us, without the W loads 40, without the LLR stores 38.
+// 30 us, without the W loads 40, without the LLR stores 38. Probes of the
+// eight-wave kernel at 2 tiles (event times, each including about 4 us of
+// launch): y loads alone 21.9 us, y loads and MFMAs 31.2, with the W loads
+// 34.1, everything 45.1; MFMAs alone (no loads) 23.0 at a 2.3 GHz shader
+// clock (1.9 GHz under the whole kernel). FP32 MFMAs hold the SIMD: a wave's
+// VALU work waits out the MFMA bursts of the other waves on its SIMD (the
+// FP32 matrix rate equals the FP32 vector rate), so a column tile costs
+// every SIMD its 48 MFMA issues per wave (32 cycles each) plus all the VALU
+// work of its waves, whatever the schedule. Two schedules built on that
+// premise measured no faster and were dropped: one 16-wave workgroup per CU
+// with y three tiles ahead and the next group's W prefetched (48.0 us), and
+// a split into 8 multiplying waves (y three tiles ahead, no other global
+// access) and 4 output waves that reduce, demodulate, store and copy the
+// next group's W into LDS (47.7 us, bit-identical; instrumented, a tile took
+// about 5700 shader cycles of which 2700 were the multiplying waves' MFMAs,
+// the rest the output waves' VALU work on the same SIMDs).
 hipError_t launch_equalize_wide(const EqParams& p0, uint32_t tc_override, uint32_t nw_override,
                                 uint32_t persistent, hipStream_t st, const char** why) {
   EqParams p = p0;
