@@ -429,9 +429,10 @@ def run_cell(args, name, sel=None, cpu=True):
     llr1 = torch.empty(S * U * lay.ue_stride, dtype=torch.int16, device=dev)
     x0 = x[0].cpu().numpy()
     evm = {}
-    for k in forms:
-        if k.endswith(DERIVED):
-            continue
+    # the sequential forms, and the base form of every derived one (same
+    # kernels) even where only the derived form was timed
+    bases = dict.fromkeys(k.rsplit("_", 1)[0] if k.endswith(DERIVED) else k for k in forms)
+    for k in bases:
         xh.zero_()
         if k == "fused":
             m.detect_slots(csi[0], y[0], llr1, xhat=xh, stream=stream, **kw1)
@@ -448,7 +449,7 @@ def run_cell(args, name, sel=None, cpu=True):
         if k.endswith(DERIVED):  # same kernels as the sequential form
             evm[k] = evm[k.rsplit("_", 1)[0]]
     evm64 = synth.evm_db(fp64_xhat(cfg, csi[:1], y[:1])[0], x0)
-    ok = {k: abs(v - evm64) < 0.1 and chunk_same.get(k, True) for k, v in evm.items()}
+    ok = {k: abs(evm[k] - evm64) < 0.1 and chunk_same.get(k, True) for k in forms}
 
     # value = the fastest FP32 form within 0.1 dB of FP64's EVM; the chosen
     # (fastest of all) form and every form's ratio to the plain drop-in beside it
