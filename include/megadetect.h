@@ -66,7 +66,9 @@ int mega_tuning_reload(void);
 
 /* Replaces Beamform::Beamform(num_bs, num_ue, batch_count)
  * (src/operators/beamform/beamform.cu:19-31). Allocates the per-group status
- * array on the current device. */
+ * array on the current device. Users: up to 128 for the FP32 form, up to 64
+ * for MEGA_FORM_MIXED (the reference's cuSOLVER path takes any number);
+ * beyond that MEGA_ERR_UNSUPPORTED. */
 int mega_beamform_create(uint32_t num_bs, uint32_t num_ue, uint32_t batch_count,
                          mega_beamform_t *out);
 

@@ -55,8 +55,8 @@ int mega_beamform_create(uint32_t num_bs, uint32_t num_ue, uint32_t batch_count,
   *out = nullptr;
   if (num_bs == 0 || num_ue == 0 || batch_count == 0)
     return fail(MEGA_ERR_INVALID_ARG, "num_bs, num_ue and batch_count must be > 0");
-  if (num_ue > mega::detail::kMaxUsers)
-    return fail(MEGA_ERR_UNSUPPORTED, "num_ue > 64 is not supported");
+  if (num_ue > mega::detail::kMaxUsersFp32)
+    return fail(MEGA_ERR_UNSUPPORTED, "num_ue > 128 is not supported");
   auto* bf = new mega_beamform{num_bs, num_ue, batch_count, 0, nullptr, nullptr, 0};
   hipError_t e = hipGetDevice(&bf->device);
   if (e == hipSuccess) e = hipMalloc(&bf->info, sizeof(int32_t) * batch_count);
@@ -101,8 +101,10 @@ static int zf_checked(uint32_t num_bs, uint32_t num_ue, uint32_t batch,
   if (!in || !out) return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
   if (num_ue == 0 || num_bs == 0)
     return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": zero size");
-  if (num_ue > mega::detail::kMaxUsers)
-    return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + ": num_ue > 64");
+  if (num_ue > (form == MEGA_FORM_FP32 ? mega::detail::kMaxUsersFp32 : mega::detail::kMaxUsers))
+    return fail(MEGA_ERR_UNSUPPORTED, std::string(who) + (form == MEGA_FORM_FP32
+                                                              ? ": num_ue > 128"
+                                                              : ": num_ue > 64 (mixed form)"));
   const char* why = nullptr;
   hipError_t e =
       form == MEGA_FORM_MIXED
@@ -208,8 +210,10 @@ static int eq_args_checked(const mega_equalize_desc* d, mega::detail::EqualizeAr
     return fail(MEGA_ERR_INVALID_ARG, std::string(who) + ": NULL buffer");
   if (d->sc_group == 0 || d->cb_size == 0)
     return fail(MEGA_ERR_INVALID_ARG, "sc_group and cb_size must be > 0");
-  if (d->num_ue > mega::detail::kMaxUsers)
-    return fail(MEGA_ERR_UNSUPPORTED, "num_ue > 64");
+  if (d->num_ue > (d->form == MEGA_FORM_FP32 ? mega::detail::kMaxUsersFp32
+                                             : mega::detail::kMaxUsers))
+    return fail(MEGA_ERR_UNSUPPORTED, d->form == MEGA_FORM_FP32 ? "num_ue > 128"
+                                                                : "num_ue > 64 (mixed form)");
   if (d->num_slots > 65535 || d->num_syms > 65535)
     return fail(MEGA_ERR_UNSUPPORTED, "more than 65535 slots or symbols per launch");
   a.csi = static_cast<const float2*>(d->csi);

@@ -153,7 +153,8 @@ int tune(TuneKey k);
 void tuning_reload();
 
 // Limits of the kernels (checked by the C ABI before launching).
-constexpr uint32_t kMaxUsers = 64;
+constexpr uint32_t kMaxUsers = 64;       // the chosen forms, precode, batch
+constexpr uint32_t kMaxUsersFp32 = 128;  // the FP32 ZF and equalize
 
 // Records a message for mega_last_error and returns code (capi.cpp).
 int set_error(int code, const std::string& msg);

@@ -7,11 +7,12 @@ namespace mega {
 namespace detail {
 namespace {
 
-// Threads per group: one wave for up to 16 users, four waves otherwise; for
-// 17 to 32 users MEGA_ZF_NTH = 512 spreads a group over 8 waves (tuning
-// sweeps).
+// Threads per group: one wave for up to 16 users, four waves up to 64, 16
+// waves for 65 to 128 (the sweep's 16 entries a thread and the Gram's 36
+// tiles); for 17 to 32 users MEGA_ZF_NTH = 512 spreads a group over 8 waves
+// (tuning sweeps).
 template <int UT>
-constexpr int zf_threads() { return UT == 1 ? 64 : 256; }
+constexpr int zf_threads() { return UT == 1 ? 64 : UT == 8 ? 1024 : 256; }
 
 template <int UT, bool A_LDS, int NTH>  // U <= 16 * UT
 __global__ __launch_bounds__(NTH) void zf_mfma(
@@ -116,7 +117,8 @@ hipError_t launch_zf(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
   if (U <= 16) return launch_zf_ut<1>(U, B, batch, in, out, info, st);
   if (U <= 32) return launch_zf_ut<2>(U, B, batch, in, out, info, st);
   if (U <= 64) return launch_zf_ut<4>(U, B, batch, in, out, info, st);
-  *why = "more than 64 users";
+  if (U <= 128) return launch_zf_ut<8>(U, B, batch, in, out, info, st);
+  *why = "more than 128 users";
   return hipErrorInvalidValue;
 }
 

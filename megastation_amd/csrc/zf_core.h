@@ -322,7 +322,7 @@ __device__ __forceinline__ void zf_gram(const float2* A, uint32_t U, uint32_t B,
   };
 
   // ---- 1. Gram ---------------------------------------------------------------
-  if (!Plan::SPLIT) {
+  if constexpr (!Plan::SPLIT) {
     for (int tile = wave; tile < Plan::NT; tile += NW) {
       int mi, mj;
       tile_of(tile, mi, mj);
@@ -468,7 +468,55 @@ __device__ __forceinline__ void zf_apply(const float2* A, uint32_t U, uint32_t B
   // A wave owns the antenna tiles wave, wave+NW, ...; only its own columns
   // are read (ahead, from global memory) and overwritten, so W may replace A
   // in place.
-  if constexpr (A_LDS) {
+  if constexpr (UT >= 8) {
+    // 65 to 128 users (1024 threads, 128 registers a lane): one A buffer, the
+    // user tiles two at a time; the same products and sums as apply_tile
+    for (uint32_t nb = wave; nb < nb_tiles; nb += NW) {
+      fetch_a(nb, abuf[0]);
+      const uint32_t b = 16 * nb + i16;
+#pragma unroll
+      for (int s = 0; s < KSU; ++s)
+        if (4u * s + q4 >= U) abuf[0][s] = make_float2(0.f, 0.f);
+      // the -G^-1 reads are the same for every antenna tile: an offset the
+      // compiler cannot see through (always 0) keeps it from holding all of
+      // them in registers across tiles; the pointer itself stays sh.G's
+      uint32_t zero = 0;
+      __asm__ __volatile__("" : "+s"(zero));
+      const float2* rs = res + zero;
+#pragma unroll
+      for (int m0 = 0; m0 < UT; m0 += 2) {
+        v4f rr[2], ri[2];
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi) {
+          v4f wr1 = {0, 0, 0, 0}, wr2 = {0, 0, 0, 0}, wi1 = {0, 0, 0, 0}, wi2 = {0, 0, 0, 0};
+#pragma unroll
+          for (int s = 0; s < KSU; ++s) {
+            const float2 m = rs[(16 * (m0 + mi) + i16) * LDG + 4 * s + q4];
+            const float gr = -m.x, gi = -m.y;
+            wr1 = mfma4(gr, abuf[0][s].x, wr1);
+            wr2 = mfma4(-gi, abuf[0][s].y, wr2);
+            wi1 = mfma4(gr, abuf[0][s].y, wi1);
+            wi2 = mfma4(gi, abuf[0][s].x, wi2);
+          }
+          rr[mi] = wr1 + wr2;
+          ri[mi] = wi1 + wi2;
+        }
+        if (b < B) {
+#pragma unroll
+          for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const uint32_t u = 16 * (m0 + mi) + 4 * q4 + r;
+              if (u < U) {
+                const float2 v = make_float2(rr[mi][r], ri[mi][r]);
+                w[u + (uint64_t)ld_w * b] = v;
+                if (w2) w2[u + (uint64_t)U * b] = v;
+              }
+            }
+        }
+      }
+    }
+  } else if constexpr (A_LDS) {
     for (uint32_t nb = wave; nb < nb_tiles; nb += NW) {
       fetch_a(nb, abuf[0]);
       apply_tile(nb, abuf[0]);

@@ -30,13 +30,17 @@ int main() {
   // beamform / ZF
   mega_beamform_t h = nullptr;
   EXPECT(mega_beamform_create(0, 4, 2, &h) == MEGA_ERR_INVALID_ARG);
-  EXPECT(mega_beamform_create(128, 65, 2, &h) == MEGA_ERR_UNSUPPORTED);
+  EXPECT(mega_beamform_create(256, 129, 2, &h) == MEGA_ERR_UNSUPPORTED);
   EXPECT(mega_beamform_create(8, 4, 2, nullptr) == MEGA_ERR_INVALID_ARG);
   EXPECT(mega_beamform_uplink(nullptr, nullptr, 1, nullptr) == MEGA_ERR_INVALID_ARG);
   EXPECT(mega_beamform_destroy(nullptr) == MEGA_OK);
   EXPECT(mega_zf(8, 4, 1, nullptr, nullptr, nullptr, 0, nullptr) == MEGA_ERR_INVALID_ARG);
   EXPECT(mega_zf(8, 4, 1, fake, fake, nullptr, 7, nullptr) == MEGA_ERR_UNSUPPORTED);
   EXPECT(mega_zf_split(8, 65, 1, fake, fake, nullptr, nullptr, nullptr, nullptr) ==
+         MEGA_ERR_UNSUPPORTED);
+  EXPECT(mega_zf(256, 65, 1, fake, fake, nullptr, MEGA_FORM_MIXED, nullptr) ==
+         MEGA_ERR_UNSUPPORTED);
+  EXPECT(mega_zf(256, 129, 1, fake, fake, nullptr, MEGA_FORM_FP32, nullptr) ==
          MEGA_ERR_UNSUPPORTED);
 
   // equalize: bad modulation order, NULL buffers, zero group size

@@ -41,7 +41,13 @@ def test_bad_beamform_arguments():
     L = m.lib()
     h = ctypes.c_void_p()
     assert L.mega_beamform_create(0, 4, 2, ctypes.byref(h)) == m.MEGA_ERR_INVALID_ARG
-    assert L.mega_beamform_create(128, 65, 2, ctypes.byref(h)) == m.MEGA_ERR_UNSUPPORTED
+    assert L.mega_beamform_create(256, 129, 2, ctypes.byref(h)) == m.MEGA_ERR_UNSUPPORTED
+    # 65 to 128 users: the FP32 form only
+    assert L.mega_zf(256, 65, 1, ctypes.c_void_p(1), ctypes.c_void_p(1), None, m.FORM_MIXED,
+                     None) == m.MEGA_ERR_UNSUPPORTED
+    assert "mixed form" in m.last_error()
+    assert L.mega_zf(256, 129, 1, ctypes.c_void_p(1), ctypes.c_void_p(1), None, m.FORM_FP32,
+                     None) == m.MEGA_ERR_UNSUPPORTED
     assert L.mega_beamform_uplink(None, None, 1, None) == m.MEGA_ERR_INVALID_ARG
     assert L.mega_zf(8, 4, 1, None, None, None, 0, None) == m.MEGA_ERR_INVALID_ARG
     assert L.mega_zf(8, 4, 1, ctypes.c_void_p(1), ctypes.c_void_p(1), None, 7,

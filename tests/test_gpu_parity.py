@@ -68,7 +68,10 @@ def test_beamformer_reference_fixture(golden):
 @pytest.mark.parametrize("U,B,G,channel", [(4, 8, 3, "iid"), (16, 64, 75, "iid"),
                                            (32, 256, 40, "kronecker"),
                                            (8, 24, 17, "iid"), (64, 256, 6, "iid"),
-                                           (5, 300, 9, "iid")])
+                                           (5, 300, 9, "iid"),
+                                           # 65 to 128 users: 16-wave workgroups
+                                           (128, 256, 5, "iid"), (96, 384, 4, "kronecker"),
+                                           (65, 130, 3, "iid")])
 def test_zf_matches_fp64_oracle(U, B, G, channel):
     print(f"case: zf U={U} B={B} G={G} {channel}", flush=True)
     rng = np.random.Generator(np.random.Philox(U * 1000 + B))
@@ -90,10 +93,11 @@ def test_zf_matches_fp64_oracle(U, B, G, channel):
     assert np.array_equal(host_c64(d_in), a)       # out-of-place leaves input intact
 
 
-def test_zf_in_place_and_failed_groups():
-    print("case: zf in place with rank-deficient groups", flush=True)
+@pytest.mark.parametrize("U,B", [(8, 32), (100, 200)])
+def test_zf_in_place_and_failed_groups(U, B):
+    print(f"case: zf in place with rank-deficient groups U={U} B={B}", flush=True)
     rng = np.random.Generator(np.random.Philox(9))
-    U, B, G = 8, 32, 10
+    G = 10
     H = synth.channel_iid(rng, G, B, U)
     H[3, :, 5] = H[3, :, 1]
     H[7] = 0
@@ -146,6 +150,8 @@ EQ_CASES = [
     (3, 5, 37, 7, 6, "standalone"),
     (20, 200, 500, 12, 6, "ldpc"),          # wide form: 20 users, 200 antennas
     (32, 96, 300, 7, 4, "standalone"),      # wide form: a part-filled antenna slice
+    (128, 256, 144, 12, 4, "standalone"),   # 128 users: four user groups of 32
+    (80, 128, 96, 16, 6, "ldpc"),           # a part-filled user group
 ]
 
 
@@ -325,7 +331,7 @@ def _detect(csi, y, U, B, N, sg, Q, lay, slots, S, want_w=True, want_x=True):
 @pytest.mark.parametrize("U,B,N,sg,Q", [(16, 64, 1200, 16, 4), (8, 64, 624, 12, 8),
                                         (16, 128, 600, 16, 6), (12, 192, 300, 12, 2),
                                         (16, 256, 240, 16, 4), (5, 24, 100, 7, 6),
-                                        (32, 128, 480, 16, 6)])
+                                        (32, 128, 480, 16, 6), (128, 256, 96, 12, 4)])
 def test_detect_slots_equals_zf_then_equalize(U, B, N, sg, Q, tuning):
     """Fused kernel == ZF kernel followed by the equalize kernel with W in
     registers, bit for bit (same arithmetic and summation order), and x_hat /
