@@ -60,9 +60,30 @@ struct EqParams {
   uint64_t ue_stride, cb_size, cb_stride, zeros, valid_bits;
   uint64_t csi_slot, y_sym, y_slot, llr_sym, llr_slot;
   uint32_t cb_size32;  // cb_size (the host checks N*Q and cb_size fit 32 bits)
+  uint32_t sg_inv, cb_inv;  // div_inv(sg), div_inv(cb_size32)
+  uint32_t last_inv;        // div_inv of the last group's subcarriers
   uint32_t store32;    // every LLR offset even and the base 4-byte aligned
   uint32_t Q;          // modulation order (used by the runtime-Q epilogue, Q = 0)
 };
+
+// n / d for any 32-bit n by a multiply: inv = div_inv(d) = ceil(2^32 / d)
+// (0 for d = 1). n * inv / 2^32 = n / d + n * e / 2^32 with 0 <= e < 1, so
+// its integer part is the quotient or one above it, and one comparison
+// corrects it. The compiler's expansion of a division by a value it does not
+// know is about 18 vector instructions.
+__host__ __device__ constexpr uint32_t div_inv(uint32_t d) {
+  return d <= 1 ? 0u : (uint32_t)(((1ull << 32) + d - 1) / d);
+}
+__device__ __forceinline__ uint32_t div_by(uint32_t n, uint32_t d, uint32_t inv) {
+  if (inv == 0) return n;  // d = 1
+  const uint32_t q = __umulhi(n, inv);
+  return (uint64_t)q * d > n ? q - 1 : q;
+}
+// Symbol of column n of a group of nk subcarriers: nk is sg except in a
+// partial last group.
+__device__ __forceinline__ uint32_t group_sym(const EqParams& p, uint32_t n, uint32_t nk) {
+  return div_by(n, nk, nk == p.sg ? p.sg_inv : p.last_inv);
+}
 
 // Where the Q LLRs of subcarrier k go within a user row
 // (equalize_kernel.cuh:461-478), and whether they can be written as Q/2
@@ -80,7 +101,7 @@ __device__ __forceinline__ LlrSlot llr_slot(const EqParams& p, uint32_t k) {
   const uint32_t q = Q > 0 ? (uint32_t)Q : p.Q;
   LlrSlot s;
   s.bit = k * q;
-  const uint32_t cb = s.bit / p.cb_size32;
+  const uint32_t cb = div_by(s.bit, p.cb_size32, p.cb_inv);
   s.off = (uint64_t)cb * p.cb_stride + p.zeros + (s.bit - cb * p.cb_size32);
   s.full = p.store32 && s.bit + q <= p.valid_bits && s.off + q <= p.ue_stride;
   return s;
@@ -132,7 +153,7 @@ __device__ __forceinline__ EqCol eq_col(const EqParams& p, uint32_t slot, uint32
   const uint32_t n = 16 * t + (threadIdx.x & 15);
   EqCol c;
   c.valid = n < nk * p.S;
-  const uint32_t sym = c.valid ? n / nk : 0;
+  const uint32_t sym = c.valid ? group_sym(p, n, nk) : 0;
   const uint32_t k = k0 + (c.valid ? n - sym * nk : 0);
   c.ls = llr_slot<Q>(p, k);
   c.rows = p.llr + slot * p.llr_slot + sym * p.llr_sym;
@@ -228,7 +249,7 @@ __device__ __forceinline__ void eq_load_tile(const EqParams& p, uint32_t slot,
   const uint32_t nk = min(p.sg, p.N - k0);
   const uint32_t n = 16 * t + i16;
   const bool valid_n = n < nk * p.S;
-  const uint32_t sym = valid_n ? n / nk : 0;
+  const uint32_t sym = valid_n ? group_sym(p, n, nk) : 0;
   const uint32_t kk = valid_n ? n - sym * nk : 0;
   const float2* yc = p.y + slot * p.y_slot + k0 + sym * p.y_sym + kk;
 #pragma unroll
@@ -335,7 +356,7 @@ __device__ __forceinline__ void eq_unit_stream(const EqParams& p, uint32_t slot,
   const uint32_t k0 = g * p.sg;
   const uint32_t nk = min(p.sg, N - k0);
   const uint32_t n = min(16 * t + i16, nk * p.S - 1);
-  const uint32_t sym = n / nk;
+  const uint32_t sym = group_sym(p, n, nk);
   const float2* ycol = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk) + q4 * N;
   const float2* W = p.csi + slot * p.csi_slot + (uint64_t)g * U * B;
   const float2* wrow[MT];

@@ -152,6 +152,9 @@ hipError_t fill_eq_params(const EqualizeArgs& a, EqParams& p, const char** why) 
   p.llr_sym = a.llr_sym;
   p.llr_slot = a.llr_slot;
   p.cb_size32 = (uint32_t)a.cb_size;
+  p.sg_inv = div_inv(a.sg);
+  p.last_inv = div_inv(a.N - (a.G - 1) * a.sg);
+  p.cb_inv = div_inv(p.cb_size32);
   p.Q = a.Q;
   p.store32 = ((reinterpret_cast<uintptr_t>(a.llr) & 3) == 0) &&
               (a.ue_stride % 2 == 0) && (a.cb_stride % 2 == 0) &&

@@ -44,7 +44,7 @@ __device__ __forceinline__ void eq_mixed_acc(const EqParams& p, const uint32_t* 
   for (int c = 0; c < T; ++c) {
     // columns past the group re-read its last one (their results are dropped)
     const uint32_t n = min(16 * (t0 + c) + i16, nk * p.S - 1);
-    const uint32_t sym = n / nk;
+    const uint32_t sym = group_sym(p, n, nk);
     ycol[c] = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
   }
   const uint32_t* Wg = Ws + 2 * (slot * p.csi_slot + (uint64_t)g * U * Bp);
@@ -248,7 +248,7 @@ __device__ __forceinline__ void eq_unit_mixed_reg(const EqParams& p,
   }
   auto ycol = [&](uint32_t t) {  // column pointer of tile t (clamped into the chunk)
     const uint32_t n = min(16 * min(t, t1 - 1) + i16, nk * p.S - 1);
-    const uint32_t sym = n / nk;
+    const uint32_t sym = group_sym(p, n, nk);
     return p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
   };
   float2 yr[R][4];

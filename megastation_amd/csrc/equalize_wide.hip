@@ -78,7 +78,7 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void equalize_wide(const 
   // addresses; their products are discarded or multiplied by W = 0)
   auto load_y = [&](uint32_t t, float2 (&yd)[KS]) {
     const uint32_t n = min(16 * t + i16, ncol - 1);
-    const uint32_t sym = n / nk;
+    const uint32_t sym = group_sym(p, n, nk);
     const float2* yc = p.y + slot * p.y_slot + sym * p.y_sym + k0 + (n - sym * nk);
 #pragma unroll
     for (int s = 0; s < KS; ++s) yd[s] = yc[(uint64_t)min(bw + 4 * s + q4, B - 1) * N];
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void equalize_wide_p(cons
   };
   auto load_y = [&](const Where& x, float2 (&yd)[KS]) {
     const uint32_t n = min(16 * x.t + i16, x.ncol - 1);
-    const uint32_t sym = n / x.nk;
+    const uint32_t sym = group_sym(p, n, x.nk);
     const float2* yc = p.y + x.slot * p.y_slot + sym * p.y_sym + x.k0 + (n - sym * x.nk);
 #pragma unroll
     for (int s = 0; s < KS; ++s) yd[s] = yc[(uint64_t)min(bw + 4 * s + q4, B - 1) * N];

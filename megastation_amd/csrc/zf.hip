@@ -34,21 +34,28 @@ __global__ __launch_bounds__(NTH) void zf_mfma(
 template <int UT, int NTH>
 hipError_t launch_zf_nth(uint32_t U, uint32_t B, uint32_t batch, const float2* in,
                          float2* out, int32_t* info, hipStream_t st) {
-  const size_t with_a = zf_dyn_lds<UT, NTH>(U, B, true);
-  const bool a_lds = UT == 1 && sizeof(ZfShared<UT>) + with_a <= 120 * 1024;
-  const size_t lds = a_lds ? with_a : zf_dyn_lds<UT, NTH>(U, B, false);
-  const void* fn = a_lds ? reinterpret_cast<const void*>(&zf_mfma<UT, true, NTH>)
-                         : reinterpret_cast<const void*>(&zf_mfma<UT, false, NTH>);
+  if constexpr (UT == 1) {
+    const size_t with_a = zf_dyn_lds<UT, NTH>(U, B, true);
+    if (sizeof(ZfShared<UT>) + with_a <= 120 * 1024) {
+      if (with_a > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zf_mfma<UT, true, NTH>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)with_a);
+        if (e != hipSuccess) return e;
+      }
+      hipLaunchKernelGGL((zf_mfma<UT, true, NTH>), dim3(batch), dim3(NTH), with_a, st, in, out,
+                         info, U, B);
+      return hipGetLastError();
+    }
+  }
+  const size_t lds = zf_dyn_lds<UT, NTH>(U, B, false);
   if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zf_mfma<UT, false, NTH>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  if (a_lds)
-    hipLaunchKernelGGL((zf_mfma<UT, true, NTH>), dim3(batch), dim3(NTH), lds, st, in, out, info,
-                       U, B);
-  else
-    hipLaunchKernelGGL((zf_mfma<UT, false, NTH>), dim3(batch), dim3(NTH), lds, st, in, out,
-                       info, U, B);
+  hipLaunchKernelGGL((zf_mfma<UT, false, NTH>), dim3(batch), dim3(NTH), lds, st, in, out, info,
+                     U, B);
   return hipGetLastError();
 }
 
