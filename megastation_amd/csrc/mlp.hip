@@ -15,9 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-#include <map>
-#include <mutex>
 
+#include "internal.h"
 #include "mlp.h"
 
 namespace mega {
@@ -221,11 +220,11 @@ uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t)
 }  // namespace
 
 // K splits: enough workgroups to cover the CUs (>= 256) while every split
-// keeps at least 8 K steps. The partial sums live in a workspace owned by the
-// library per (device, stream): GEMMs queued on one stream run in order and
-// share it safely, GEMMs on different streams never share one. It grows on
-// demand with hipFree/hipMalloc, so a split GEMM is refused while its stream
-// is being captured into a graph.
+// keeps at least 8 K steps. The partial sums live in the library's workspace
+// of the (device, stream) (stream_workspace): work queued on one stream runs
+// in order and shares it safely, streams never share one. It grows on demand
+// with hipFree/hipMalloc, so a split GEMM is refused while its stream is
+// being captured into a graph.
 hipError_t mlp_gemm(const MlpGemm& g, hipStream_t st) {
   if (g.M == 0 || g.N == 0) return hipSuccess;
   const uint32_t tiles = ((g.N + TN - 1) / TN) * ((g.M + TM - 1) / TM);
@@ -236,33 +235,11 @@ hipError_t mlp_gemm(const MlpGemm& g, hipStream_t st) {
   splits = (g.K + kchunk - 1) / kchunk;
   float* part = nullptr;
   if (splits > 1) {
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    hipError_t e = hipStreamIsCapturing(st, &cap);
+    void* w = nullptr;
+    const hipError_t e =
+        stream_workspace(st, (size_t)splits * g.M * g.N * sizeof(float), &w);
     if (e != hipSuccess) return e;
-    if (cap != hipStreamCaptureStatusNone) return hipErrorStreamCaptureUnsupported;
-    struct Ws {
-      void* p = nullptr;
-      size_t bytes = 0;
-    };
-    static std::mutex mu;
-    static std::map<std::pair<int, hipStream_t>, Ws> ws;
-    int dev = 0;
-    e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    const size_t need = (size_t)splits * g.M * g.N * sizeof(float);
-    std::lock_guard<std::mutex> lock(mu);
-    Ws& w = ws[{dev, st}];
-    if (w.bytes < need) {
-      if (w.p) {
-        e = hipFree(w.p);  // synchronizes: earlier GEMMs are done with it
-        if (e != hipSuccess) return e;
-        w = Ws{};
-      }
-      e = hipMalloc(&w.p, need);
-      if (e != hipSuccess) return e;
-      w.bytes = need;
-    }
-    part = static_cast<float*>(w.p);
+    part = static_cast<float*>(w);
   }
   const dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
   hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, st, g.A, g.sai, g.sak, g.B, g.sbk, g.sbj,
