@@ -31,11 +31,16 @@ __device__ __forceinline__ uint32_t find_cell(const uint32_t* prefix, uint32_t n
   return lo;
 }
 
+// Threads per group, as mega_zf's: one wave up to 16 users, four up to 32,
+// 16 for 33 to 64 (the same W bits on any of these counts).
+template <int UT>
+constexpr int cells_threads() { return UT == 1 ? 64 : UT == 2 ? 256 : 1024; }
+
 template <int UT, bool A_LDS>
-__global__ __launch_bounds__(UT == 1 ? 64 : 256) void zf_cells(const BatchCell* cells,
-                                                               const uint32_t* prefix,
-                                                               uint32_t ncells) {
-  constexpr int NTH = UT == 1 ? 64 : 256;
+__global__ __launch_bounds__(cells_threads<UT>()) void zf_cells(const BatchCell* cells,
+                                                                const uint32_t* prefix,
+                                                                uint32_t ncells) {
+  constexpr int NTH = cells_threads<UT>();
   __shared__ ZfShared<UT> sh;
   extern __shared__ float sDyn[];
   const uint32_t c = find_cell(prefix, ncells, blockIdx.x);
@@ -122,21 +127,27 @@ __global__ void reset_counters(uint32_t* c, uint32_t n) {
   if (i < n) c[i] = 0;
 }
 
+// A is staged in LDS for one-wave groups only (the class planner's choice).
 template <int UT>
 hipError_t prepare_zf_ut(const ZfClassLaunch& L) {
-  const void* fn = L.a_lds ? reinterpret_cast<const void*>(&zf_cells<UT, true>)
-                           : reinterpret_cast<const void*>(&zf_cells<UT, false>);
   if (L.lds <= 65536) return hipSuccess;
+  const void* fn = reinterpret_cast<const void*>(&zf_cells<UT, false>);
+  if constexpr (UT == 1)
+    if (L.a_lds) fn = reinterpret_cast<const void*>(&zf_cells<UT, true>);
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds);
 }
 
 template <int UT>
 hipError_t launch_zf_ut(const ZfClassLaunch& L, hipStream_t st) {
-  const dim3 grid(L.blocks), block(UT == 1 ? 64 : 256);
-  if (L.a_lds)
-    hipLaunchKernelGGL((zf_cells<UT, true>), grid, block, L.lds, st, L.cells, L.prefix, L.ncells);
-  else
-    hipLaunchKernelGGL((zf_cells<UT, false>), grid, block, L.lds, st, L.cells, L.prefix, L.ncells);
+  const dim3 grid(L.blocks), block(cells_threads<UT>());
+  if constexpr (UT == 1) {
+    if (L.a_lds) {
+      hipLaunchKernelGGL((zf_cells<UT, true>), grid, block, L.lds, st, L.cells, L.prefix,
+                         L.ncells);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((zf_cells<UT, false>), grid, block, L.lds, st, L.cells, L.prefix, L.ncells);
   return hipGetLastError();
 }
 
@@ -209,7 +220,7 @@ size_t zf_class_lds(uint32_t UT, uint32_t U, uint32_t B, bool a_lds) {
   switch (UT) {
     case 1: return zf_dyn_lds<1, 64>(U, B, a_lds);
     case 2: return zf_dyn_lds<2, 256>(U, B, a_lds);
-    default: return zf_dyn_lds<4, 256>(U, B, a_lds);
+    default: return zf_dyn_lds<4, cells_threads<4>()>(U, B, a_lds);
   }
 }
 

@@ -36,7 +36,7 @@ constexpr TuneSpec kSpecs[kTuneCount] = {
     {"MEGA_PREC_ACC", 0, 8, 8},   // precode: 8 = 64 antennas per wave
     {"MEGA_DET_PF", 1, 0, 1},     // fused detect: next-tile y prefetch
     {"MEGA_EQ_WTC", 0, 1, 1 << 20},  // FP32 wide form: column tiles per workgroup
-    {"MEGA_ZF_NTH", 0, 256, 512},   // FP32 ZF, 17-32 users: threads per group
+    {"MEGA_ZF_NTH", 0, 256, 512},   // FP32 ZF threads per group: 512 (17-32 users), 256 (33-64)
     {"MEGA_ZF_WIDE", 0, 1, 2},      // FP32 ZF: 1 one workgroup per group, 2 three kernels
     {"MEGA_UPLINK_STREAMS", 0, 1, 2},  // uplink data path: 1 one stream, 2 two streams
     {"MEGA_EQ_WNW", 0, 4, 8},       // FP32 wide form: waves per workgroup (4 or 8)

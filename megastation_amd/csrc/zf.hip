@@ -7,12 +7,15 @@ namespace mega {
 namespace detail {
 namespace {
 
-// Threads per group: one wave for up to 16 users, four waves up to 64, 16
-// waves for 65 to 128 (the sweep's 16 entries a thread and the Gram's 36
-// tiles); for 17 to 32 users MEGA_ZF_NTH = 512 spreads a group over 8 waves
-// (tuning sweeps).
+// Threads per group: one wave for up to 16 users, four waves up to 32, 16
+// waves for 33 to 128 (at 64 users the sweep keeps 4 entries a thread, and
+// the Gram's 10 tiles and the apply's antenna tiles each go to a wave of
+// their own; W is bit-identical to four waves' and the launch at 256 x 64,
+// 273 groups, takes 125 us against 166, at 48 users 113 against 142).
+// MEGA_ZF_NTH overrides for tuning sweeps: 512 for 17 to 32 users, 256 for
+// 33 to 64.
 template <int UT>
-constexpr int zf_threads() { return UT == 1 ? 64 : UT == 8 ? 1024 : 256; }
+constexpr int zf_threads() { return UT == 1 ? 64 : UT == 2 ? 256 : 1024; }
 
 template <int UT, bool A_LDS, int NTH>  // U <= 16 * UT
 __global__ __launch_bounds__(NTH) void zf_mfma(
@@ -65,6 +68,10 @@ hipError_t launch_zf_ut(uint32_t U, uint32_t B, uint32_t batch, const float2* in
   if constexpr (UT == 2) {
     const int nth = tune(kZfNth);
     if (nth == 512) return launch_zf_nth<2, 512>(U, B, batch, in, out, info, st);
+  }
+  if constexpr (UT == 4) {
+    const int nth = tune(kZfNth);
+    if (nth == 256) return launch_zf_nth<4, 256>(U, B, batch, in, out, info, st);
   }
   return launch_zf_nth<UT, zf_threads<UT>()>(U, B, batch, in, out, info, st);
 }

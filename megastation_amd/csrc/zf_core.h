@@ -50,13 +50,15 @@ struct ZfShared {
 };
 
 // Gram work split: NT lower-triangle tiles x NCH antenna chunks of 64. When a
-// workgroup has more waves than tiles, the chunks of a tile are spread over
-// the waves and their partial sums go through LDS; otherwise a wave sums its
-// tile's chunks in registers, in the same order.
+// workgroup of up to 8 waves has more waves than tiles, the chunks of a tile
+// are spread over the waves and their partial sums go through LDS; otherwise
+// a wave sums its tile's chunks in registers, in the same order (16-wave
+// workgroups, at 128 registers a lane, have no room for the split's two
+// chunk buffers).
 template <int UT, int NW>
 struct ZfPlan {
   static constexpr int NT = UT * (UT + 1) / 2;
-  static constexpr bool SPLIT = NW > NT;
+  static constexpr bool SPLIT = NW > NT && NW <= 8;
 };
 
 constexpr uint32_t kGramChunk = 16;  // K steps (4 antennas each) per chunk
@@ -468,9 +470,9 @@ __device__ __forceinline__ void zf_apply(const float2* A, uint32_t U, uint32_t B
   // A wave owns the antenna tiles wave, wave+NW, ...; only its own columns
   // are read (ahead, from global memory) and overwritten, so W may replace A
   // in place.
-  if constexpr (UT >= 8) {
-    // 65 to 128 users (1024 threads, 128 registers a lane): one A buffer, the
-    // user tiles two at a time; the same products and sums as apply_tile
+  if constexpr (NTH >= 1024) {
+    // 16-wave workgroups (128 registers a lane): one A buffer, the user
+    // tiles two at a time; the same products and sums as apply_tile
     for (uint32_t nb = wave; nb < nb_tiles; nb += NW) {
       fetch_a(nb, abuf[0]);
       const uint32_t b = 16 * nb + i16;

@@ -93,6 +93,35 @@ def test_zf_matches_fp64_oracle(U, B, G, channel):
     assert np.array_equal(host_c64(d_in), a)       # out-of-place leaves input intact
 
 
+@pytest.mark.parametrize("U,B,G", [(64, 256, 40), (40, 200, 17)])
+def test_zf_thread_counts_are_bit_identical(U, B, G, tuning):
+    """33 to 64 users on 4 or 16 waves per group (MEGA_ZF_NTH): the Gram's
+    chunks are added in the same order, every sweep entry and apply product
+    is the same arithmetic, so W and the status are the same bits; both
+    against the FP64 oracle."""
+    print(f"case: zf thread counts U={U} B={B} G={G}", flush=True)
+    rng = np.random.Generator(np.random.Philox(U * 7 + B))
+    H = synth.channel_kronecker(rng, G, B, U, 0.5, 0.3)
+    H[3] = 0                                           # one failed group
+    a = np.conj(H).astype(np.complex64)
+    _, w64, info64 = oracle.zf(a, U, B, G, "f64")
+    t = torch()
+    outs = []
+    for nth in (256, 0):                              # 0: the default, 16 waves
+        tuning(MEGA_ZF_NTH=nth)
+        d_out = t.zeros(G * B * U, dtype=t.complex64, device="cuda:0")
+        d_info = t.full((G,), -7, dtype=t.int32, device="cuda:0")
+        m.zf(dev_c64(a), d_out, B, U, G, info=d_info)
+        _sync()
+        outs.append((host_c64(d_out), d_info.cpu().numpy()))
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][1], info64)
+    ok = info64 == 0
+    w = outs[0][0].reshape(G, B * U).astype(np.complex128)[ok]
+    ref = w64.reshape(G, B * U)[ok]
+    assert (np.linalg.norm(w - ref, axis=1) / np.linalg.norm(ref, axis=1)).max() < 1e-4
+
+
 @pytest.mark.parametrize("U,B", [(8, 32), (100, 200)])
 def test_zf_in_place_and_failed_groups(U, B):
     print(f"case: zf in place with rank-deficient groups U={U} B={B}", flush=True)
