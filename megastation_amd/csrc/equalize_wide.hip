@@ -318,12 +318,13 @@ bool eq_wide_ok(uint32_t U, uint32_t B) { return U >= 1 && U <= 32 && B > 32 && 
 // eight-wave kernel at 2 tiles (event times, each including about 4 us of
 // launch): y loads alone 21.9 us, y loads and MFMAs 31.2, with the W loads
 // 34.1, everything 45.1; MFMAs alone (no loads) 23.0 at a 2.3 GHz shader
-// clock (1.9 GHz under the whole kernel). FP32 MFMAs hold the SIMD: a wave's
-// VALU work waits out the MFMA bursts of the other waves on its SIMD (the
-// FP32 matrix rate equals the FP32 vector rate), so a column tile costs
-// every SIMD its 48 MFMA issues per wave (32 cycles each) plus all the VALU
-// work of its waves, whatever the schedule. Two schedules built on that
-// premise measured no faster and were dropped: one 16-wave workgroup per CU
+// clock (1.9 GHz under the whole kernel). Within a tile, FP32 MFMAs and VALU
+// work serialize on a SIMD: a wave's VALU work waits out the MFMA burst of
+// another wave on its SIMD (the FP32 matrix rate equals the FP32 vector
+// rate); in aggregate no unit is saturated (matrix cores busy 0.28 of the
+// SIMD-cycles, VALU 0.08), the time being each tile's chain of loads, MFMAs,
+// reduction, demod and stores. Two schedules that overlap more of that chain
+// measured no faster and were dropped: one 16-wave workgroup per CU
 // with y three tiles ahead and the next group's W prefetched (48.0 us), and
 // a split into 8 multiplying waves (y three tiles ahead, no other global
 // access) and 4 output waves that reduce, demodulate, store and copy the
