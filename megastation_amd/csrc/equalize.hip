@@ -171,7 +171,7 @@ hipError_t launch_equalize(const EqualizeArgs& a, hipStream_t st,
   EqParams p;
   hipError_t e = fill_eq_params(a, p, why);
   if (e != hipSuccess) return e;
-  // Wide cells (17 to 32 users, 128 to 256 antennas, e.g. 256 x 32): the
+  // Wide cells (17 to 64 users, 128 to 256 antennas, e.g. 256 x 32): the
   // antennas split over a workgroup's waves with W in registers
   // (equalize_wide.hip); MEGA_EQ_MODE=3 forces that form wherever it applies.
   const int mode = tune(kEqMode);

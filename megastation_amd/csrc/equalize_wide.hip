@@ -1,7 +1,8 @@
-// FP32 equalize + demod for wide cells (up to 32 users, 128 or 256
-// antennas, e.g. 256 x 32): the antennas are split over the four waves of a
-// workgroup, each wave keeps its slice of W_g in registers for the whole
-// unit, and the complex products take three real MFMAs instead of four.
+// FP32 equalize + demod for wide cells (up to 64 users, 128 or 256
+// antennas, e.g. 256 x 32): the antennas are split over the four or eight
+// waves of a workgroup, each wave keeps its slice of W_g in registers for
+// the whole unit (16, 32 or 64 users: one, two or four 16-user tiles), and
+// the complex products take three real MFMAs instead of four.
 //
 // Unit = (slot, group, chunk of TC column tiles of 16 (symbol, subcarrier)
 // columns). Wave w owns antennas 4*KS*w .. 4*KS*(w+1)-1 (KS steps of 4) and
@@ -33,7 +34,7 @@ namespace detail {
 namespace {
 
 template <int MT, int KS, int NW, int Q>
-__global__ __launch_bounds__(64 * NW, NW == 8 ? 4 : 2) void equalize_wide(const EqParams p,
+__global__ __launch_bounds__(64 * NW, NW == 8 && MT <= 2 ? 4 : 2) void equalize_wide(const EqParams p,
                                                                            uint32_t chunks,
                                                             uint32_t tc, uint32_t units) {
   // partial x tiles: [buffer][wave][16-user tile][re, im][row r][lane]
@@ -301,7 +302,7 @@ hipError_t launch_q(const EqParams& p, uint32_t chunks, uint32_t tc, uint32_t un
 
 }  // namespace
 
-bool eq_wide_ok(uint32_t U, uint32_t B) { return U >= 1 && U <= 32 && B > 32 && B <= 256; }
+bool eq_wide_ok(uint32_t U, uint32_t B) { return U >= 1 && U <= 64 && B > 32 && B <= 256; }
 
 // Column tiles per workgroup: the most that still give every CU five
 // workgroups (1280 in the grid), so each wave's W slice serves as many tiles
@@ -334,7 +335,7 @@ bool eq_wide_ok(uint32_t U, uint32_t B) { return U >= 1 && U <= 32 && B > 32 && 
 hipError_t launch_equalize_wide(const EqParams& p0, uint32_t tc_override, uint32_t nw_override,
                                 uint32_t persistent, hipStream_t st, const char** why) {
   EqParams p = p0;
-  const uint32_t MT = p.U <= 16 ? 1 : 2;
+  const uint32_t MT = p.U <= 16 ? 1 : p.U <= 32 ? 2 : 4;
   // waves per workgroup: 4 (antennas 32 or 64 per wave) or 8 (16 or 32);
   // eight by default for rows of more than 128 antennas (four waves per SIMD
   // at 126 VGPRs against two at 234: 44.8 against 49.2 us at 256 x 32)
@@ -343,7 +344,7 @@ hipError_t launch_equalize_wide(const EqParams& p0, uint32_t tc_override, uint32
                       : p.B > 128                         ? 8
                                                           : 4;
   const uint32_t KS = (p.B <= 128 ? 8 : 16) * 4 / NW;
-  if (persistent >= 1) {
+  if (persistent >= 1 && MT <= 2) {
     // persistent[k]: k * 256 workgroups (k = 2: two per CU at four waves per
     // SIMD), never more than the tiles
     const uint64_t total = (uint64_t)p.slots * p.G * p.col_tiles;
@@ -358,8 +359,14 @@ hipError_t launch_equalize_wide(const EqParams& p0, uint32_t tc_override, uint32
                    : launch_p<2, 8, 8>(p, wgs, (uint32_t)total, p.Q, st);
   }
   const uint64_t groups = (uint64_t)p.slots * p.G;
+  // 1280 units: five workgroups per CU; 33 to 64 users (one workgroup per
+  // CU, 128 KB of partials) 640. Measured with four user tiles at 256 x 64
+  // (273 groups of 10 tiles): 4 tiles a unit 82.8 us, 2 85.2, 1 96.0, 10
+  // 108.6, the streamed plan 87.6; at 256 x 48 79.8 against 83.1, at
+  // 128 x 64 49.5 against 53.7.
+  const uint64_t min_units = MT == 4 ? 640 : 1280;
   uint32_t tc = 1;
-  while (tc < p.col_tiles && groups * ((p.col_tiles + tc) / (tc + 1)) >= 1280) ++tc;
+  while (tc < p.col_tiles && groups * ((p.col_tiles + tc) / (tc + 1)) >= min_units) ++tc;
   if (tc_override >= 1) tc = tc_override < p.col_tiles ? tc_override : p.col_tiles;
   const uint32_t chunks = (p.col_tiles + tc - 1) / tc;
   const uint64_t units = groups * chunks;
@@ -368,6 +375,9 @@ hipError_t launch_equalize_wide(const EqParams& p0, uint32_t tc_override, uint32
     return hipErrorInvalidValue;
   }
   const uint32_t n = (uint32_t)units;
+  if (MT == 4)  // 33 to 64 users: eight waves, two a SIMD (a 64-register W slice)
+    return p.B <= 128 ? launch_q<4, 4, 8>(p, chunks, tc, n, p.Q, st)
+                      : launch_q<4, 8, 8>(p, chunks, tc, n, p.Q, st);
   if (NW == 8) {
     if (MT == 1) return KS == 4 ? launch_q<1, 4, 8>(p, chunks, tc, n, p.Q, st)
                                 : launch_q<1, 8, 8>(p, chunks, tc, n, p.Q, st);

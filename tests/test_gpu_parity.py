@@ -180,6 +180,7 @@ EQ_CASES = [
     (20, 200, 500, 12, 6, "ldpc"),          # wide form: 20 users, 200 antennas
     (32, 96, 300, 7, 4, "standalone"),      # wide form: a part-filled antenna slice
     (128, 256, 144, 12, 4, "standalone"),   # 128 users: four user groups of 32
+    (48, 256, 240, 12, 6, "ldpc"),          # wide form, four user tiles, the last empty
     (80, 128, 96, 16, 6, "ldpc"),           # a part-filled user group
 ]
 
