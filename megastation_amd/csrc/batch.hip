@@ -36,8 +36,9 @@ __device__ __forceinline__ uint32_t find_cell(const uint32_t* prefix, uint32_t n
 template <int UT>
 constexpr int cells_threads() { return UT == 1 ? 64 : UT == 2 ? 256 : 1024; }
 
+// 17-32 users: three four-wave groups a CU, as in zf_mfma
 template <int UT, bool A_LDS>
-__global__ __launch_bounds__(cells_threads<UT>()) void zf_cells(const BatchCell* cells,
+__global__ __launch_bounds__(cells_threads<UT>(), UT == 2 ? 3 : 1) void zf_cells(const BatchCell* cells,
                                                                 const uint32_t* prefix,
                                                                 uint32_t ncells) {
   constexpr int NTH = cells_threads<UT>();

@@ -17,8 +17,13 @@ namespace {
 template <int UT>
 constexpr int zf_threads() { return UT == 1 ? 64 : UT == 2 ? 256 : 1024; }
 
+// Four-wave groups of 17-32 users are held to 168 registers a lane (no
+// spills), so three groups fit a CU instead of two: a ZF launch beside an
+// equalize launch, or over two slots' 546 groups, keeps more of them resident
+// (256x32 slot pipeline 0.066 against 0.077 ms a slot; one 273-group launch
+// alone unchanged).
 template <int UT, bool A_LDS, int NTH>  // U <= 16 * UT
-__global__ __launch_bounds__(NTH) void zf_mfma(
+__global__ __launch_bounds__(NTH, UT == 2 && NTH == 256 ? 3 : 1) void zf_mfma(
     const float2* in, float2* out, int32_t* info, uint32_t U, uint32_t B) {
   __shared__ ZfShared<UT> sh;
   extern __shared__ float sDyn[];
