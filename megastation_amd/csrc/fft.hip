@@ -23,7 +23,10 @@
 // radix-2 decimation-in-time stages run in place, four per barrier (radix
 // 2^4, sixteen elements per thread in registers); the store reads the
 // shifted bins straight from LDS. Sizes: powers of
-// two from 16 to 4096 (51 KB of LDS at 4096).
+// two from 16 to 4096 (51 KB of LDS at 4096); 2048 and 4096 points run an
+// instantiation with the size fixed at compile time (the same arithmetic,
+// with the LDS addresses folded into immediate offsets).
+#include "eq_core.h"
 #include "internal.h"
 
 namespace mega {
@@ -37,7 +40,7 @@ constexpr uint32_t kFftMax = 4096;
 // element every 32, so power-of-two strides across lanes (the samples' in the
 // early passes, the twiddles' in the later ones: up to 128 entries apart)
 // spread over the banks.
-__device__ __forceinline__ uint32_t pidx(uint32_t i) { return i + (i >> 5); }
+__host__ __device__ constexpr uint32_t pidx(uint32_t i) { return i + (i >> 5); }
 
 // u + w v and u - w v (complex)
 __device__ __forceinline__ float2 bfly_add(float2 u, float2 w, float2 v) {
@@ -77,6 +80,75 @@ __device__ __forceinline__ void fft_pass(float2* sx, const float2* tw, uint32_t 
   }
 }
 
+// The same pass with the transform size 2^LC and the first stage ST known
+// at compile time (the 2048- and 4096-point symbols): thread j's elements
+// and twiddles sit at a per-thread base plus compile-time offsets, so the
+// LDS accesses take immediate offsets instead of per-element address
+// arithmetic. pass_offsets_hold checks the offset identities for every
+// thread and element of each instantiation at compile time.
+template <int R, int LC, int ST>
+constexpr bool pass_offsets_hold() {
+  constexpr uint32_t N = 1u << LC, half = 1u << ST, J = N >> R;
+  for (uint32_t j = 0; j < J; ++j) {
+    const uint32_t k = j & (half - 1), i0 = ((j >> ST) << (ST + R)) + k;
+    for (uint32_t m = 0; m < (1u << R); ++m)
+      if (pidx(i0 + m * half) != pidx(i0) + m * half + ((m * half) >> 5)) return false;
+    for (int q = 0; q < R; ++q) {
+      const uint32_t sh = LC - 1 - ST - q;
+      for (uint32_t c = 0; c < (1u << q); ++c) {
+        const uint32_t C = (c * half) << sh;
+        if (pidx((k << sh) + C) != pidx(k << sh) + C + (C >> 5)) return false;
+      }
+    }
+  }
+  return true;
+}
+
+template <int R, int LC, int ST>
+__device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw) {
+  static_assert(pass_offsets_hold<R, LC, ST>(), "LDS offset identities");
+  constexpr int M = 1 << R;
+  constexpr uint32_t half = 1u << ST, J = (1u << LC) >> R;
+  static_assert(J <= (uint32_t)kFftThreads, "one element block per thread");
+  const uint32_t j = threadIdx.x;
+  if (J < (uint32_t)kFftThreads && j >= J) return;
+  const uint32_t k = j & (half - 1);
+  float2* const xb = sx + pidx(((j >> ST) << (ST + R)) + k);
+  float2 v[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) v[m] = xb[m * half + ((m * half) >> 5)];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int sp = 1 << q;
+    const uint32_t sh = LC - 1 - ST - q;
+    const float2* const tb = tw + pidx(k << sh);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      if (m & sp) continue;
+      const uint32_t C = ((uint32_t)(m & (sp - 1)) * half) << sh;
+      const float2 w = tb[C + (C >> 5)];
+      const float2 u = v[m], x = v[m + sp];
+      v[m] = bfly_add(u, w, x);
+      v[m + sp] = bfly_sub(u, w, x);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) xb[m * half + ((m * half) >> 5)] = v[m];
+}
+
+// Bit-reversed LDS slot of sample tid + 256 i for N = 2^LC: the reversed
+// index is (rev8(tid) << (LC - 8)) | rev(i), and the pad of the sum is the
+// pad of the first part (checked at compile time).
+template <int LC>
+constexpr bool load_offsets_hold() {
+  for (uint32_t t = 0; t < 256; ++t)
+    for (uint32_t i = 0; i < (1u << (LC - 8)); ++i) {
+      const uint32_t base = t << (LC - 8);
+      if (pidx(base + i) != pidx(base) + i) return false;
+    }
+  return true;
+}
+
 template <typename Q>
 __device__ __forceinline__ float2 load_iq(const Q* p, uint64_t n, float inv) {
   return make_float2((float)p[2 * n] * inv, (float)p[2 * n + 1] * inv);
@@ -93,14 +165,29 @@ __device__ __forceinline__ Q quantize(float v) {
   return (Q)(int)__builtin_rintf(v);
 }
 
-template <typename Q, int MODE>
+// LC: log2 of the transform size when it is fixed at compile time (11, 12),
+// else 0 and the size comes from the arguments.
+template <typename Q, int MODE, int LC>
 __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   __shared__ float2 sx[kFftMax + kFftMax / 32];
   __shared__ float2 tw[kFftMax / 2 + kFftMax / 64];
   const uint32_t b = blockIdx.x, s = blockIdx.y;
-  const uint32_t N = a.ca, L = a.log2ca, H = N / 2;
+  const uint32_t N = LC ? 1u << LC : a.ca, L = LC ? LC : a.log2ca, H = N / 2;
   const uint64_t row = (uint64_t)s * a.in_sym + (uint64_t)b * N;
-  if constexpr (MODE == kIfft) {
+  if constexpr (LC != 0) {
+    static_assert(LC >= 8 && load_offsets_hold<LC>(), "load offsets");
+    float2* const xb = sx + pidx((__builtin_bitreverse32(threadIdx.x) >> 24) << (LC - 8));
+    const float inv = 1.0f / a.qscale;
+#pragma unroll
+    for (uint32_t i = 0; i < (1u << (LC - 8)); ++i) {
+      const uint32_t n = threadIdx.x + kFftThreads * i;
+      const uint32_t ri = __builtin_bitreverse32(i) >> (32 - (LC - 8));
+      if constexpr (MODE == kIfft)
+        xb[ri] = static_cast<const float2*>(a.in)[row + n];
+      else
+        xb[ri] = load_iq(static_cast<const Q*>(a.in) + 2 * row, n, inv);
+    }
+  } else if constexpr (MODE == kIfft) {
     const float2* in = static_cast<const float2*>(a.in) + row;
     for (uint32_t n = threadIdx.x; n < N; n += kFftThreads)
       sx[pidx(__builtin_bitreverse32(n) >> (32 - L))] = in[n];
@@ -128,7 +215,17 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   // takes the 2^R elements x[i + m h] (h = 2^st, i = its block's start + k,
   // k < h) and runs stages st .. st+R-1 in registers; stage st+q pairs
   // m and m + 2^q with W_{2^(st+q+1)}^(k + (m mod 2^q) h).
-  uint32_t st = 0;
+  if constexpr (LC != 0) {
+    static_for<0, LC / 4>([&](auto pc) {
+      fft_pass_ct<4, LC, 4 * decltype(pc)::value>(sx, tw);
+      __syncthreads();
+    });
+    if constexpr (LC % 4 != 0) {
+      fft_pass_ct<LC % 4, LC, 4 * (LC / 4)>(sx, tw);
+      __syncthreads();
+    }
+  }
+  uint32_t st = LC != 0 ? L : 0;
   for (; st + 4 <= L; st += 4) {
     fft_pass<4>(sx, tw, N, L, st);
     __syncthreads();
@@ -167,14 +264,21 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   }
 }
 
-template <int MODE>
-hipError_t launch_q(const FftArgs& a, hipStream_t st) {
+template <int MODE, int LC>
+hipError_t launch_lc(const FftArgs& a, hipStream_t st) {
   const dim3 grid(a.B, a.S), block(kFftThreads);
   if (a.qbits == 16)
-    hipLaunchKernelGGL((fft_kernel<int16_t, MODE>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((fft_kernel<int16_t, MODE, LC>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((fft_kernel<int8_t, MODE>), grid, block, 0, st, a);
+    hipLaunchKernelGGL((fft_kernel<int8_t, MODE, LC>), grid, block, 0, st, a);
   return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_q(const FftArgs& a, hipStream_t st) {
+  if (a.log2ca == 12) return launch_lc<MODE, 12>(a, st);
+  if (a.log2ca == 11) return launch_lc<MODE, 11>(a, st);
+  return launch_lc<MODE, 0>(a, st);
 }
 
 }  // namespace
