@@ -52,6 +52,8 @@ def test_pilot_fft_reference_fixture(golden):
     (7, 64, 64, 3, 1, np.int16),          # last pass of two stages (6 = 4 + 2)
     (4, 128, 100, 60, 2, np.int16),       # last pass of three stages (7 = 4 + 3)
     (9, 1024, 600, 200, 2, np.int8),      # 10 = 4 + 4 + 2
+    (6, 2048, 1200, 424, 2, np.int8),     # compile-time size 2048, char samples
+    (3, 4096, 3276, 410, 1, np.int8),     # compile-time size 4096, char samples
 ])
 def test_uplink_fft_matches_oracle(B, ca, num, start, S, dtype):
     print(f"case: uplink FFT B={B} ca={ca} num={num} start={start} S={S} {dtype.__name__}",
