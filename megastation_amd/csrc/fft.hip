@@ -104,7 +104,11 @@ constexpr bool pass_offsets_hold() {
   return true;
 }
 
-template <int R, int LC, int ST>
+// The butterflies: in the first pass (k = 0) the twiddles of c = 0 are 1 and
+// those of a quarter turn are (0, -+1) (the table's own values there), so
+// w x is x or a swap; otherwise u + w x takes four FMAs and u - w x =
+// 2u - (u + w x) two more.
+template <int R, int LC, int ST, bool INV>
 __device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw) {
   static_assert(pass_offsets_hold<R, LC, ST>(), "LDS offset identities");
   constexpr int M = 1 << R;
@@ -125,11 +129,21 @@ __device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw) {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       if (m & sp) continue;
-      const uint32_t C = ((uint32_t)(m & (sp - 1)) * half) << sh;
-      const float2 w = tb[C + (C >> 5)];
+      const uint32_t c = (uint32_t)(m & (sp - 1));  // W_{2^(ST+q+1)}^(k + c half)
       const float2 u = v[m], x = v[m + sp];
-      v[m] = bfly_add(u, w, x);
-      v[m + sp] = bfly_sub(u, w, x);
+      if (ST == 0 && (c == 0 || 2 * c == (uint32_t)sp)) {
+        // k = 0: w = 1, or the quarter turn w = (0, -+1) of exp(-+2 pi i / 4)
+        const float2 t = c == 0 ? x : INV ? make_float2(-x.y, x.x) : make_float2(x.y, -x.x);
+        v[m] = make_float2(u.x + t.x, u.y + t.y);
+        v[m + sp] = make_float2(u.x - t.x, u.y - t.y);
+      } else {
+        const uint32_t C = (c * half) << sh;
+        const float2 w = tb[C + (C >> 5)];
+        const float2 a = make_float2(__builtin_fmaf(w.x, x.x, __builtin_fmaf(-w.y, x.y, u.x)),
+                                     __builtin_fmaf(w.x, x.y, __builtin_fmaf(w.y, x.x, u.y)));
+        v[m] = a;
+        v[m + sp] = make_float2(__builtin_fmaf(2.f, u.x, -a.x), __builtin_fmaf(2.f, u.y, -a.y));
+      }
     }
   }
 #pragma unroll
@@ -217,11 +231,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   // m and m + 2^q with W_{2^(st+q+1)}^(k + (m mod 2^q) h).
   if constexpr (LC != 0) {
     static_for<0, LC / 4>([&](auto pc) {
-      fft_pass_ct<4, LC, 4 * decltype(pc)::value>(sx, tw);
+      fft_pass_ct<4, LC, 4 * decltype(pc)::value, MODE == kIfft>(sx, tw);
       __syncthreads();
     });
     if constexpr (LC % 4 != 0) {
-      fft_pass_ct<LC % 4, LC, 4 * (LC / 4)>(sx, tw);
+      fft_pass_ct<LC % 4, LC, 4 * (LC / 4), MODE == kIfft>(sx, tw);
       __syncthreads();
     }
   }
