@@ -24,8 +24,9 @@
 // 2^4, sixteen elements per thread in registers); the store reads the
 // shifted bins straight from LDS. Sizes: powers of
 // two from 16 to 4096 (51 KB of LDS at 4096); 2048 and 4096 points run an
-// instantiation with the size fixed at compile time (the same arithmetic,
-// with the LDS addresses folded into immediate offsets).
+// instantiation with the size fixed at compile time: the LDS addresses are
+// immediate offsets, the first pass runs in registers on the samples as
+// loaded, and u - w x is formed as 2u - (u + w x).
 #include "eq_core.h"
 #include "internal.h"
 
@@ -104,11 +105,16 @@ constexpr bool pass_offsets_hold() {
   return true;
 }
 
-// The butterflies: in the first pass (k = 0) the twiddles of c = 0 are 1 and
-// those of a quarter turn are (0, -+1) (the table's own values there), so
-// w x is x or a swap; otherwise u + w x takes four FMAs and u - w x =
-// 2u - (u + w x) two more.
-template <int R, int LC, int ST, bool INV>
+// The butterflies: u + w x takes four FMAs and u - w x = 2u - (u + w x) two
+// more.
+__device__ __forceinline__ void bfly_fma(float2& u, float2& x, float2 w) {
+  const float2 a = make_float2(__builtin_fmaf(w.x, x.x, __builtin_fmaf(-w.y, x.y, u.x)),
+                               __builtin_fmaf(w.x, x.y, __builtin_fmaf(w.y, x.x, u.y)));
+  x = make_float2(__builtin_fmaf(2.f, u.x, -a.x), __builtin_fmaf(2.f, u.y, -a.y));
+  u = a;
+}
+
+template <int R, int LC, int ST>
 __device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw) {
   static_assert(pass_offsets_hold<R, LC, ST>(), "LDS offset identities");
   constexpr int M = 1 << R;
@@ -130,37 +136,64 @@ __device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw) {
     for (int m = 0; m < M; ++m) {
       if (m & sp) continue;
       const uint32_t c = (uint32_t)(m & (sp - 1));  // W_{2^(ST+q+1)}^(k + c half)
-      const float2 u = v[m], x = v[m + sp];
-      if (ST == 0 && (c == 0 || 2 * c == (uint32_t)sp)) {
-        // k = 0: w = 1, or the quarter turn w = (0, -+1) of exp(-+2 pi i / 4)
-        const float2 t = c == 0 ? x : INV ? make_float2(-x.y, x.x) : make_float2(x.y, -x.x);
-        v[m] = make_float2(u.x + t.x, u.y + t.y);
-        v[m + sp] = make_float2(u.x - t.x, u.y - t.y);
-      } else {
-        const uint32_t C = (c * half) << sh;
-        const float2 w = tb[C + (C >> 5)];
-        const float2 a = make_float2(__builtin_fmaf(w.x, x.x, __builtin_fmaf(-w.y, x.y, u.x)),
-                                     __builtin_fmaf(w.x, x.y, __builtin_fmaf(w.y, x.x, u.y)));
-        v[m] = a;
-        v[m + sp] = make_float2(__builtin_fmaf(2.f, u.x, -a.x), __builtin_fmaf(2.f, u.y, -a.y));
-      }
+      const uint32_t C = (c * half) << sh;
+      bfly_fma(v[m], v[m + sp], tb[C + (C >> 5)]);
     }
   }
 #pragma unroll
   for (int m = 0; m < M; ++m) xb[m * half + ((m * half) >> 5)] = v[m];
 }
 
-// Bit-reversed LDS slot of sample tid + 256 i for N = 2^LC: the reversed
-// index is (rev8(tid) << (LC - 8)) | rev(i), and the pad of the sum is the
-// pad of the first part (checked at compile time).
+// The first pass (stages 0-3) of the compile-time sizes runs in registers
+// on the samples as loaded: thread j < N/16 takes elements 16 j + m of the
+// bit-reversed sequence, the samples rev_{LC-4}(j) + 2^(LC-4) rev4(m), and
+// its twiddles W_{2^(q+1)}^c are the constants exp(-+i pi e / 8), e =
+// c 2^(3-q): 1 and the quarter turn (e = 0, 4) as a copy and a swap, the
+// others as FMAs. The result goes to LDS at 16 j + m (pad included).
+constexpr float kC1 = 0.923879532511286756f;  // cos(pi/8)
+constexpr float kS1 = 0.382683432365089772f;  // sin(pi/8)
+constexpr float kR2 = 0.707106781186547524f;  // cos(pi/4)
+constexpr float kCos16[8] = {1.f, kC1, kR2, kS1, 0.f, -kS1, -kR2, -kC1};
+constexpr float kSin16[8] = {0.f, kS1, kR2, kC1, 1.f, kC1, kR2, kS1};
+
+constexpr uint32_t rev_bits(uint32_t x, int bits) {
+  uint32_t r = 0;
+  for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+  return r;
+}
+// The gather and the LDS slots of the first pass, checked for every thread
+// and element: element 16 j + m of the bit-reversed sequence is sample
+// rev_{LC-4}(j) + 2^(LC-4) rev4(m), and its slot is pidx(16 j) + m.
 template <int LC>
-constexpr bool load_offsets_hold() {
-  for (uint32_t t = 0; t < 256; ++t)
-    for (uint32_t i = 0; i < (1u << (LC - 8)); ++i) {
-      const uint32_t base = t << (LC - 8);
-      if (pidx(base + i) != pidx(base) + i) return false;
+constexpr bool first_pass_holds() {
+  for (uint32_t j = 0; j < ((1u << LC) >> 4); ++j)
+    for (uint32_t m = 0; m < 16; ++m) {
+      if (rev_bits(16 * j + m, LC) != rev_bits(j, LC - 4) + (rev_bits(m, 4) << (LC - 4)))
+        return false;
+      if (pidx(16 * j + m) != pidx(16 * j) + m) return false;
     }
   return true;
+}
+
+template <bool INV>
+__device__ __forceinline__ void fft_first_pass(float2 (&v)[16]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int sp = 1 << q;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      if (m & sp) continue;
+      const int e = (m & (sp - 1)) << (3 - q);
+      const float2 u = v[m], x = v[m + sp];
+      if (e == 0 || e == 4) {
+        const float2 t = e == 0 ? x : INV ? make_float2(-x.y, x.x) : make_float2(x.y, -x.x);
+        v[m] = make_float2(u.x + t.x, u.y + t.y);
+        v[m + sp] = make_float2(u.x - t.x, u.y - t.y);
+      } else {
+        bfly_fma(v[m], v[m + sp], make_float2(kCos16[e], INV ? kSin16[e] : -kSin16[e]));
+      }
+    }
+  }
 }
 
 template <typename Q>
@@ -188,18 +221,22 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   const uint32_t b = blockIdx.x, s = blockIdx.y;
   const uint32_t N = LC ? 1u << LC : a.ca, L = LC ? LC : a.log2ca, H = N / 2;
   const uint64_t row = (uint64_t)s * a.in_sym + (uint64_t)b * N;
+  constexpr uint32_t J0 = LC ? (1u << LC) >> 4 : 0;  // threads of the first pass
+  static_assert(LC == 0 || (LC >= 8 && J0 <= (uint32_t)kFftThreads && first_pass_holds<LC>()),
+                "first pass");
+  float2 v0[16];
   if constexpr (LC != 0) {
-    static_assert(LC >= 8 && load_offsets_hold<LC>(), "load offsets");
-    float2* const xb = sx + pidx((__builtin_bitreverse32(threadIdx.x) >> 24) << (LC - 8));
-    const float inv = 1.0f / a.qscale;
+    if (J0 == (uint32_t)kFftThreads || threadIdx.x < J0) {
+      const uint32_t rj = __builtin_bitreverse32(threadIdx.x) >> (32 - (LC - 4));
+      const float inv = 1.0f / a.qscale;
 #pragma unroll
-    for (uint32_t i = 0; i < (1u << (LC - 8)); ++i) {
-      const uint32_t n = threadIdx.x + kFftThreads * i;
-      const uint32_t ri = __builtin_bitreverse32(i) >> (32 - (LC - 8));
-      if constexpr (MODE == kIfft)
-        xb[ri] = static_cast<const float2*>(a.in)[row + n];
-      else
-        xb[ri] = load_iq(static_cast<const Q*>(a.in) + 2 * row, n, inv);
+      for (uint32_t m = 0; m < 16; ++m) {
+        const uint32_t n = rj + ((__builtin_bitreverse32(m) >> 28) << (LC - 4));
+        if constexpr (MODE == kIfft)
+          v0[m] = static_cast<const float2*>(a.in)[row + n];
+        else
+          v0[m] = load_iq(static_cast<const Q*>(a.in) + 2 * row, n, inv);
+      }
     }
   } else if constexpr (MODE == kIfft) {
     const float2* in = static_cast<const float2*>(a.in) + row;
@@ -224,18 +261,26 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     tw[pidx(N / 4 + k)] = make_float2(-sn, sgn * cs);
     if (k > 0) tw[pidx(H - k)] = make_float2(-cs, sgn * sn);
   }
+  if constexpr (LC != 0) {
+    if (J0 == (uint32_t)kFftThreads || threadIdx.x < J0) {
+      fft_first_pass<MODE == kIfft>(v0);
+      float2* const xb = sx + pidx(16 * threadIdx.x);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) xb[m] = v0[m];  // pidx(16 j + m) = pidx(16 j) + m
+    }
+  }
   __syncthreads();
   // R stages per barrier (radix 2^R, R = 4, the last pass 1 to 3): a thread
   // takes the 2^R elements x[i + m h] (h = 2^st, i = its block's start + k,
   // k < h) and runs stages st .. st+R-1 in registers; stage st+q pairs
   // m and m + 2^q with W_{2^(st+q+1)}^(k + (m mod 2^q) h).
   if constexpr (LC != 0) {
-    static_for<0, LC / 4>([&](auto pc) {
-      fft_pass_ct<4, LC, 4 * decltype(pc)::value, MODE == kIfft>(sx, tw);
+    static_for<1, LC / 4>([&](auto pc) {
+      fft_pass_ct<4, LC, 4 * decltype(pc)::value>(sx, tw);
       __syncthreads();
     });
     if constexpr (LC % 4 != 0) {
-      fft_pass_ct<LC % 4, LC, 4 * (LC / 4), MODE == kIfft>(sx, tw);
+      fft_pass_ct<LC % 4, LC, 4 * (LC / 4)>(sx, tw);
       __syncthreads();
     }
   }
