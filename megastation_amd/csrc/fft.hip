@@ -26,7 +26,8 @@
 // two from 16 to 4096 (51 KB of LDS at 4096); 2048 and 4096 points run an
 // instantiation with the size fixed at compile time: the LDS addresses are
 // immediate offsets, the first pass runs in registers on the samples as
-// loaded, and u - w x is formed as 2u - (u + w x).
+// loaded, u - w x is formed as 2u - (u + w x), and the last pass stores its
+// bins to the outputs directly.
 #include "eq_core.h"
 #include "internal.h"
 
@@ -114,8 +115,11 @@ __device__ __forceinline__ void bfly_fma(float2& u, float2& x, float2 w) {
   u = a;
 }
 
-template <int R, int LC, int ST>
-__device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw) {
+// With LAST (the pass that ends at stage LC - 1: there j = k and the
+// elements are bins k + m half in natural order) the results go to
+// emit(bin, value) instead of back to LDS.
+template <int R, int LC, int ST, bool LAST = false, class Emit = int>
+__device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw, Emit&& emit = 0) {
   static_assert(pass_offsets_hold<R, LC, ST>(), "LDS offset identities");
   constexpr int M = 1 << R;
   constexpr uint32_t half = 1u << ST, J = (1u << LC) >> R;
@@ -140,8 +144,14 @@ __device__ __forceinline__ void fft_pass_ct(float2* sx, const float2* tw) {
       bfly_fma(v[m], v[m + sp], tb[C + (C >> 5)]);
     }
   }
+  if constexpr (LAST) {
+    static_assert(ST + R == LC, "the last pass ends the transform");
 #pragma unroll
-  for (int m = 0; m < M; ++m) xb[m * half + ((m * half) >> 5)] = v[m];
+    for (int m = 0; m < M; ++m) emit(k + m * half, v[m]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < M; ++m) xb[m * half + ((m * half) >> 5)] = v[m];
+  }
 }
 
 // The first pass (stages 0-3) of the compile-time sizes runs in registers
@@ -274,15 +284,41 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
   // takes the 2^R elements x[i + m h] (h = 2^st, i = its block's start + k,
   // k < h) and runs stages st .. st+R-1 in registers; stage st+q pairs
   // m and m + 2^q with W_{2^(st+q+1)}^(k + (m mod 2^q) h).
+  // bin X[bb] of the transform to its output: y[i] for i = (bb - shift) mod
+  // N below num (uplink), the CSI element of that subcarrier (pilot), or the
+  // quantized sample bb (IFFT)
+  const uint32_t shift = a.start + H;  // bins (i + shift) mod N, N a power of two
+  auto emit = [&](uint32_t bb, float2 e) {
+    if constexpr (MODE == kIfft) {
+      Q* o = static_cast<Q*>(a.qout) + 2 * ((uint64_t)s * a.out_sym + (uint64_t)b * N);
+      const float scale = a.qscale / (float)N;
+      o[2 * bb] = quantize<Q>(e.x * scale);
+      o[2 * bb + 1] = quantize<Q>(e.y * scale);
+    } else {
+      const uint32_t i = (bb - shift) & (N - 1);
+      if (i >= a.num) return;
+      if constexpr (MODE == kUplink) {
+        a.out[(uint64_t)s * a.out_sym + (uint64_t)b * a.num + i] = e;
+      } else {
+        const uint32_t u = a.ue_start + s * a.sg + i % a.sg;
+        if (u >= a.U) return;
+        const float2 p = a.pilot[i];
+        const uint64_t o = (uint64_t)(i / a.sg) * a.B * a.U + (uint64_t)b * a.U + u;
+        a.out[(uint64_t)s * a.out_sym + o] =
+            make_float2(e.x * p.x + e.y * p.y, e.x * p.y - e.y * p.x);
+      }
+    }
+  };
   if constexpr (LC != 0) {
-    static_for<1, LC / 4>([&](auto pc) {
+    // middle passes through LDS, the last one straight to the outputs
+    constexpr int NP = (LC + 3) / 4;  // passes: the first (in registers) .. the last
+    static_for<1, NP - 1>([&](auto pc) {
       fft_pass_ct<4, LC, 4 * decltype(pc)::value>(sx, tw);
       __syncthreads();
     });
-    if constexpr (LC % 4 != 0) {
-      fft_pass_ct<LC % 4, LC, 4 * (LC / 4)>(sx, tw);
-      __syncthreads();
-    }
+    constexpr int RL = LC - 4 * (NP - 1);
+    fft_pass_ct<RL, LC, 4 * (NP - 1), true>(sx, tw, emit);
+    return;
   }
   uint32_t st = LC != 0 ? L : 0;
   for (; st + 4 <= L; st += 4) {
@@ -299,7 +335,6 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
     fft_pass<1>(sx, tw, N, L, st);
     __syncthreads();
   }
-  const uint32_t shift = a.start + H;  // bins (i + shift) mod N, N a power of two
   if constexpr (MODE == kIfft) {
     Q* o = static_cast<Q*>(a.qout) + 2 * ((uint64_t)s * a.out_sym + (uint64_t)b * N);
     const float scale = a.qscale / (float)N;
