@@ -917,7 +917,7 @@ def main_fft(args):
         "roofline": {"bound": "hbm", "kernel": "fft_kernel (uplink, 13 symbols)",
                      "achieved": up_bytes / (up_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": up_bytes / (up_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel<short, 0>") or
+                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel<short, 0, 12>") or
                                  {}).get("traffic_bytes"),
                      "algorithmic_bytes": up_bytes, "avg_ms": up_ms},
         "forms": {"pilot_ms": pil_ms, "uplink_ms": up_ms},
@@ -1076,7 +1076,7 @@ def main_uplink(args):
         "roofline": {"bound": "hbm", "kernel": "fft_kernel (uplink, 13 symbols, staged form)",
                      "achieved": fft_bytes / (fft_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": fft_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel<short, 0>") or
+                     "traffic": (measured_traffic("u3_fft_256x32", "fft_kernel<short, 0, 12>") or
                                  {}).get("traffic_bytes"),
                      "algorithmic_bytes": fft_bytes, "avg_ms": fft_ms},
         "forms": {"staged": {"ms_per_step": ms_staged, "symbols_per_s": symbols / (ms_staged * 1e-3)},
