@@ -300,10 +300,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_kernel(const FftArgs a) {
       if constexpr (MODE == kUplink) {
         a.out[(uint64_t)s * a.out_sym + (uint64_t)b * a.num + i] = e;
       } else {
-        const uint32_t u = a.ue_start + s * a.sg + i % a.sg;
+        const uint32_t gi = div_by(i, a.sg, a.sg_inv);  // group of subcarrier i
+        const uint32_t u = a.ue_start + s * a.sg + (i - gi * a.sg);
         if (u >= a.U) return;
         const float2 p = a.pilot[i];
-        const uint64_t o = (uint64_t)(i / a.sg) * a.B * a.U + (uint64_t)b * a.U + u;
+        const uint64_t o = (uint64_t)gi * a.B * a.U + (uint64_t)b * a.U + u;
         a.out[(uint64_t)s * a.out_sym + o] =
             make_float2(e.x * p.x + e.y * p.y, e.x * p.y - e.y * p.x);
       }
@@ -399,6 +400,7 @@ hipError_t launch_fft(const FftArgs& a0, int mode, hipStream_t st, const char** 
   }
   if (mode == kPilot && (a.sg == 0 || a.U == 0)) return hipSuccess;
   a.log2ca = 31 - __builtin_clz(a.ca);
+  a.sg_inv = div_inv(a.sg);
   a.qscale = a.qbits == 16 ? 32768.0f : 128.0f;
   if (mode == kIfft) return launch_q<kIfft>(a, st);
   return mode == kPilot ? launch_q<kPilot>(a, st) : launch_q<kUplink>(a, st);

@@ -117,6 +117,7 @@ struct FftArgs {
   uint64_t in_sym, out_sym;
   uint32_t log2ca;  // set by the launcher
   float qscale;     // set by the launcher
+  uint32_t sg_inv;  // set by the launcher: div_inv(sg), pilot only
 };
 // mode: 0 uplink, 1 pilot, 2 downlink IFFT
 hipError_t launch_fft(const FftArgs& a, int mode, hipStream_t stream, const char** why);
