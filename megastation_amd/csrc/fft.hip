@@ -176,12 +176,14 @@ constexpr uint32_t rev_bits(uint32_t x, int bits) {
 // rev_{LC-4}(j) + 2^(LC-4) rev4(m), and its slot is pidx(16 j) + m.
 template <int LC>
 constexpr bool first_pass_holds() {
-  for (uint32_t j = 0; j < ((1u << LC) >> 4); ++j)
-    for (uint32_t m = 0; m < 16; ++m) {
-      if (rev_bits(16 * j + m, LC) != rev_bits(j, LC - 4) + (rev_bits(m, 4) << (LC - 4)))
-        return false;
-      if (pidx(16 * j + m) != pidx(16 * j) + m) return false;
-    }
+  if constexpr (LC >= 8) {  // the sizes with a first pass in registers
+    for (uint32_t j = 0; j < ((1u << LC) >> 4); ++j)
+      for (uint32_t m = 0; m < 16; ++m) {
+        if (rev_bits(16 * j + m, LC) != rev_bits(j, LC - 4) + (rev_bits(m, 4) << (LC - 4)))
+          return false;
+        if (pidx(16 * j + m) != pidx(16 * j) + m) return false;
+      }
+  }
   return true;
 }
 
